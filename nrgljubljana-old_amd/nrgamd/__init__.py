@@ -1,3 +1,3 @@
 """Python side of the MI355X NRG engine: table derivation, `data` generation,
 and the ctypes binding of the C-ABI library (used by tests and bench.py)."""
-from . import symtab  # noqa: F401
+from . import symtab, datagen  # noqa: F401
