@@ -1,0 +1,142 @@
+// nrg_host.h - host side of the MI355X NRG engine (drop-in for the `nrg`
+// binary of NRG Ljubljana).  Only small vectors and descriptors live here;
+// every matrix of the path is a device buffer driven through nrg_gpu.h.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "nrg_gpu.h"
+
+namespace nrg {
+
+// Invariant subspace label: (Q, SS) for QS, (Q, SSZ) for QSZ.  std::pair's
+// lexicographic order is the order of Invar::operator< (src/invar.cc:198-200),
+// which fixes every loop order of the reference.
+using Invar = std::pair<int, int>;
+using Twoinvar = std::pair<Invar, Invar>;
+
+[[noreturn]] void die(const std::string &msg);
+
+// ---- parameters (src/param.cc defaults, src/misc.h:71-124 parser) --------
+struct Params {
+    std::map<std::string, std::string> raw;
+    double Lambda = 1.0, z = 1.0, bandrescale = 1.0, keepenergy = -1.0, T = -999.0, betabar = 1.0;
+    std::string discretization = "Y", strategy = "kept", specd;
+    int keep = 100, keepmin = 0, bins = 100, safeguardmax = 200, Nmax = 0, Ninit = 0;
+    int width_td = 16, prec_td = 10, width_custom = 16, prec_custom = 10, prec_xy = 10;
+    bool dmnrg = false, fdm = false, dm = false, savebins = false, broaden = true, NN2even = true,
+         NN2avg = false, NN1 = false, lastall = false, lastalloverride = false, calc0 = true,
+         dumpenergies = false;
+    double goodE = 2.0, NNtanh = 0.0, safeguard = 1e-5, fixeps = 1e-15, emin = -1.0, emax = -1.0,
+           discard_trim = 1e-16, discard_immediately = 1e-16, alpha = 0.3, omega0 = -1.0,
+           omega0_ratio = 1.0, broaden_max = 2.5, broaden_min = -99.0, broaden_min_ratio = 3.0,
+           broaden_ratio = 1.05;
+    void load(const std::string &path);
+};
+
+// ---- `data` file (src/nrg.cc:3293-3401, src/read-input.cc:98-144) -------
+struct HostMat {
+    int rows = 0, cols = 0;
+    std::vector<double> v;
+};
+using HostOp = std::map<Twoinvar, HostMat>;
+
+struct Data {
+    std::string sym;
+    int channels = 0, Nmax = 0;
+    std::map<Invar, std::vector<double>> energies;
+    std::vector<HostOp> f;  // per channel
+    std::map<std::string, HostOp> ops_s, ops_d;
+    double GS = 0.0;
+    std::vector<std::vector<double>> xi, zeta;  // per channel, 0..Nmax
+    void load(const std::string &path);
+};
+
+// ---- coefficient tables (derived by nrgamd/symtab.py) --------------------
+struct TabEntry {
+    int a, b, ch;
+    std::string kind;
+    std::vector<double> vals;
+    double at(int ss) const;  // value at SS (QS) / constant (QSZ)
+};
+
+struct Tables {
+    std::string sym;
+    int channels = 0, combs = 0, ssmax = 0;
+    std::vector<Invar> In, QN;
+    std::vector<std::vector<int>> occ;
+    std::vector<TabEntry> offdiag, recalcf, doublet;
+    void load(const std::string &path);
+    int mult(const Invar &I) const { return sym == "QS" ? I.second : 1; }
+    bool allowed(const Invar &I) const { return sym == "QS" ? I.second > 0 : true; }
+    bool triangle(const Invar &a, const Invar &b, const Invar &c) const;
+    double specdens_factor(const Invar &Ip, const Invar &I1) const;
+};
+
+// ---- device helpers -------------------------------------------------------
+struct Gpu {
+    nrg_ctx *ctx = nullptr;
+    explicit Gpu(int device);
+    ~Gpu();
+    void check(nrg_status s, const char *what) const;
+    double *alloc(size_t n);
+    void free(void *p);
+    void h2d(double *d, const double *h, size_t n);
+    void d2h(double *h, const double *d, size_t n);
+};
+
+// Bump allocator over large device chunks, released together.
+struct Arena {
+    Gpu *gpu = nullptr;
+    std::vector<double *> chunks;
+    double *cur = nullptr;
+    size_t left = 0;
+    size_t chunk = size_t(1) << 22;  // doubles (32 MiB); grows to fit
+    explicit Arena(Gpu *g) : gpu(g) {}
+    ~Arena() { release(); }
+    double *alloc(size_t n);
+    void release();
+};
+
+struct DevMat {
+    double *p = nullptr;
+    int rows = 0, cols = 0, ld = 0;
+};
+using DevOp = std::map<Twoinvar, DevMat>;
+
+// One invariant subspace of one iteration, with its (pre-truncation)
+// eigenvectors resident on the device for the density-matrix sweeps.
+struct Sub {
+    Invar I;
+    std::vector<Invar> anc;
+    std::vector<int> rmax, off;
+    int d = 0, kept = 0;
+    double *U = nullptr;  // d x d, row r = eigenvector r
+    double *W = nullptr;  // device eigenvalues (after shift/fix for use as E_prev)
+    double *Eabs = nullptr;  // device absolute energies - GSenergy (pass 2)
+    std::vector<double> value, absenergy;
+};
+
+struct Iter {
+    int N = 0;
+    double scale = 0.0, Egs = 0.0;
+    std::map<Invar, Sub> subs;
+    std::unique_ptr<Arena> arena;  // U, W
+};
+
+struct Options {
+    std::string workdir = ".";
+    std::string dump_eigen;  // binary dump of eigenvalues / kept counts
+    std::string stats;       // JSON with stage timings
+    std::string tables_dir;
+    int device = 0;
+};
+
+int run(const Options &opt);
+
+}  // namespace nrg

@@ -230,8 +230,10 @@ class Bins:
         E, W = E[keep], W[keep]
         if E.size == 0:
             return
-        pos = (np.log10(E) - self.l10min) * self.bpd
-        idx = pos.astype(np.int64)  # truncation toward zero, like long(pos)
+        with np.errstate(divide="ignore"):
+            pos = (np.log10(E) - self.l10min) * self.bpd
+        # truncation toward zero, like long(pos); E == 0 gives -inf -> first bin
+        idx = np.where(pos > -9e18, np.trunc(np.where(pos > -9e18, pos, 0.0)), -1).astype(np.int64)
         low = idx < 0
         high = idx >= self.nr - 1
         mid = ~(low | high)

@@ -1,0 +1,69 @@
+"""Host driver (C++) against the oracle, with the C ABI served by a CPU stand-in.
+
+These tests exercise everything the host decides (subspace lists, truncation,
+thermodynamics, recalculation and density-matrix task lists, spectrum
+merging, output formats) without a GPU.  The GPU-side arithmetic is tested
+separately in test_gpu_*.py through the real library.
+"""
+import os
+
+import pytest
+
+from conftest import load_oracle
+import nrgio
+
+CASES = {
+    "qs_fdm": dict(sym="QS", nch=1, Nmax=10, param=dict(keep=80, keepenergy=8.0, T=0.01, fdm=True,
+                                                       specd="A_d-A_d", bins=50, savebins=True)),
+    "qs_dmnrg": dict(sym="QS", nch=1, Nmax=10, param=dict(keep=80, keepenergy=8.0, T=1e-3, dmnrg=True,
+                                                         specd="A_d-A_d", bins=50, savebins=True)),
+    "qsz_fdm": dict(sym="QSZ", nch=1, Nmax=8, param=dict(keep=80, keepenergy=8.0, T=0.01, fdm=True,
+                                                        specd="A_d-A_d-u A_d-A_d-d", bins=50, savebins=True)),
+    "qs2ch_fdm": dict(sym="QS", nch=2, Nmax=5, Lambda=3.0,
+                      model_kw=dict(U=0.5, eps_d=-0.25, Gamma=0.02),
+                      param=dict(keep=120, keepenergy=6.0, T=0.01, fdm=True, specd="A_d-A_d", bins=50,
+                                 savebins=True)),
+}
+
+
+def _spectra(p):
+    out = []
+    for name in p["specd"].split():
+        parts = name.split("-")
+        spin = {"u": 1, "d": -1}.get(parts[2], 0) if len(parts) == 3 else 0
+        for kind in (["DMNRG"] if p.get("dmnrg") else []) + (["FDM"] if p.get("fdm") else []):
+            out.append((kind, parts[0], parts[1], spin))
+    return out
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_host_matches_oracle(tmp_path, case):
+    c = CASES[case]
+    wd = str(tmp_path)
+    nrgio.make_case(wd, sym=c["sym"], nch=c["nch"], Nmax=c["Nmax"], Lambda=c.get("Lambda", 2.0),
+                    model_kw=c.get("model_kw"), **c["param"])
+    nrgio.run_engine(nrgio.build_mock(), wd)
+    o = load_oracle().Oracle(wd).run()
+    nrgio.compare_with_oracle(wd, o, spectra=_spectra(c["param"]))
+    assert os.path.exists(os.path.join(wd, "DONE"))
+
+
+def test_param_duplicate_is_fatal(tmp_path):
+    wd = str(tmp_path)
+    nrgio.make_case(wd, Nmax=4, keep=50, T=0.1)
+    with open(os.path.join(wd, "param"), "a") as f:
+        f.write("keep=60\n")
+    with pytest.raises(RuntimeError):
+        nrgio.run_engine(nrgio.build_mock(), wd)
+
+
+def test_missing_table_entry_is_an_error(tmp_path):
+    wd = str(tmp_path)
+    nrgio.make_case(wd, Nmax=4, keep=50, T=0.1)
+    with open(os.path.join(wd, "data")) as f:
+        txt = f.read()
+    # a data file whose Nmax disagrees with its coefficient tables
+    with open(os.path.join(wd, "param"), "a") as f:
+        f.write("Nmax=6\n")
+    with pytest.raises(RuntimeError):
+        nrgio.run_engine(nrgio.build_mock(), wd)
