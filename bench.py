@@ -1,0 +1,185 @@
+"""Benchmark of the NRG hot path on one MI355X.
+
+A step is one complete NRG solve of the configured workload through the
+drop-in engine (libnrg_engine.so, called in-process): NRG pass with batched
+assembly + eigensolve + recalculation, DM/FDM backward sweeps, and the DMNRG
+pass with operator recalculation and spectral binning.  The input `data` file
+is generated before the timed region.
+
+Prints one JSON line: value = eigenpairs/s (sum over iterations and
+subspaces of the eigenpairs produced, divided by the run time), plus the
+roofline of the dominant kernel (HIP events on the launch stream, algorithmic
+work counted by the library) and a CPU baseline (the numpy oracle on a bounded
+sample of the same workload, i.e. LAPACK dsyevd + BLAS on the host cores).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "nrgljubljana-old_amd")
+sys.path.insert(0, PKG)
+
+PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and MFMA)
+PEAK_HBM_GBS = 8000.0
+
+CONFIGS = {
+    # SIAM QS, keep <= 500 states (Ecut = 8 omega_N), Nmax = 30, DM-NRG spectrum at T=1e-8
+    "siam_qs_keep500_dmnrg": dict(model="siam", sym="QS", nch=1, Lambda=2.0, Nmax=30,
+                                  model_kw=dict(U=1.0, eps_d=-0.5, Gamma=0.04),
+                                  param=dict(keep=500, keepenergy=8.0, T=1e-8, dmnrg=True, specd="A_d-A_d",
+                                             bins=50, savebins=True)),
+    # SIAM QS, keep <= 4000 states (Ecut = 10 omega_N), Nmax = 50, FDM spectrum + thermodynamics
+    "siam_qs_keep4000_fdm": dict(model="siam", sym="QS", nch=1, Lambda=2.0, Nmax=50,
+                                 model_kw=dict(U=1.0, eps_d=-0.5, Gamma=0.04),
+                                 param=dict(keep=4000, keepenergy=10.0, T=1e-10, fdm=True, specd="A_d-A_d",
+                                            bins=100, savebins=True)),
+    # two-channel Anderson QS, keep 10000, Lambda = 3, Nmax = 40, FDM spectrum + susceptibility
+    "anderson2ch_qs_keep10000_fdm": dict(model="siam", sym="QS", nch=2, Lambda=3.0, Nmax=40,
+                                         model_kw=dict(U=0.5, eps_d=-0.25, Gamma=0.02),
+                                         param=dict(keep=10000, keepenergy=100.0, T=1e-10, fdm=True,
+                                                    specd="A_d-A_d", bins=100, savebins=True)),
+}
+DEFAULT = "siam_qs_keep4000_fdm"
+
+
+def make_inputs(wd, cfg, Nmax=None):
+    from nrgamd import datagen
+    N = Nmax or cfg["Nmax"]
+    m = datagen.siam(sym=cfg["sym"], nch=cfg["nch"], Lambda=cfg["Lambda"], **cfg["model_kw"])
+    m.write(os.path.join(wd, "data"), cfg["Lambda"], 1.0, N)
+    datagen.write_param(os.path.join(wd, "param"), Lambda=cfg["Lambda"], **cfg["param"])
+
+
+def ensure_tables():
+    tab = os.path.join(PKG, "tables", "qs-1ch.tab")
+    if not os.path.exists(tab):
+        import subprocess
+        subprocess.check_call([sys.executable, "-m", "nrgamd.symtab", os.path.join(PKG, "tables")], cwd=PKG)
+
+
+def run_engine(lib, wd, stats):
+    """One solve in-process: chdir into the work directory and call nrg_main."""
+    args = [b"nrg", b"--stats", stats.encode(), b"--tables", os.path.join(PKG, "tables").encode()]
+    argv = (ctypes.c_char_p * len(args))(*args)
+    cwd = os.getcwd()
+    os.chdir(wd)
+    try:
+        rc = lib.nrg_main(len(args), argv)
+    finally:
+        os.chdir(cwd)
+    if rc != 0:
+        raise RuntimeError("nrg_main returned %d" % rc)
+    with open(stats) as f:
+        return json.load(f)
+
+
+def cpu_baseline(cfg, budget_s=20.0):
+    """Oracle (numpy/LAPACK) on the first shells of the same workload."""
+    sys.path.insert(0, ROOT)
+    from oracle import nrg_oracle
+    try:
+        from threadpoolctl import threadpool_info
+        cores = max([p.get("num_threads", 1) for p in threadpool_info()] + [1])
+    except Exception:
+        cores = os.cpu_count() or 1
+    N = 4
+    best = None
+    while True:
+        with tempfile.TemporaryDirectory() as wd:
+            make_inputs(wd, cfg, Nmax=N)
+            t0 = time.time()
+            o = nrg_oracle.Oracle(wd).run()
+            dt = time.time() - t0
+        pairs = sum(len(v) for it in o.iters for v, k in it.values())
+        best = (pairs / dt, N, dt, pairs)
+        if dt > budget_s / 3 or N >= cfg["Nmax"]:
+            break
+        N = min(cfg["Nmax"], N + max(2, N // 2))
+    rate, N, dt, pairs = best
+    return {"value": rate, "unit": "eigenpairs/s", "cores": cores, "kind": "port",
+            "sample": "numpy oracle (LAPACK eigh + BLAS), both passes, first %d of %d shells: %d eigenpairs in %.1f s"
+                      % (N, cfg["Nmax"], pairs, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default=DEFAULT, choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % a.gpus)
+        sys.exit(2)
+    cfg = CONFIGS[a.config]
+    ensure_tables()
+    import nrgamd.capi as capi
+    gpulib = capi.load()
+    eng = ctypes.CDLL(os.path.join(PKG, "lib", "libnrg_engine.so"))
+    eng.nrg_main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]
+    with tempfile.TemporaryDirectory() as wd:
+        make_inputs(wd, cfg)
+        stats = os.path.join(wd, "stats.json")
+        for w in range(a.warmup):
+            print("warmup %d" % w, file=sys.stderr, flush=True)
+            run_engine(eng, wd, stats)
+        gpulib.nrg_timing_enable(1)
+        gpulib.nrg_timing_reset()
+        t0 = time.perf_counter()
+        pairs = 0
+        stage_sum = {}
+        for s in range(a.steps):
+            print("step %d" % s, file=sys.stderr, flush=True)
+            st = run_engine(eng, wd, stats)
+            pairs += st["eigenpairs"]
+            for k, v in st["stages"].items():
+                stage_sum[k] = stage_sum.get(k, 0.0) + v["s"]
+        t1 = time.perf_counter()
+        kst = capi.kernel_stats(gpulib)
+    total = t1 - t0
+    value = pairs / total
+    dom = max(kst.items(), key=lambda kv: kv[1]["ms"]) if kst else None
+    roof = None
+    if dom:
+        name, k = dom
+        if k["flops"] > 0:
+            ach = k["flops"] / (k["ms"] * 1e-3) / 1e12
+            roof = {"kernel": name, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": None,
+                    "avg_launch_ms": k["ms"] / max(1, k["launches"])}
+        else:
+            ach = k["bytes"] / (k["ms"] * 1e-3) / 1e9
+            roof = {"kernel": name, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": ach / PEAK_HBM_GBS, "traffic": None,
+                    "avg_launch_ms": k["ms"] / max(1, k["launches"])}
+    cpu = None if a.no_cpu_baseline else cpu_baseline(cfg)
+    out = {
+        "metric": "NRG eigenpairs/s (full two-pass solve)",
+        "value": value,
+        "unit": "eigenpairs/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * total / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated SIAM data file)",
+        "config": {"workload": a.config, "run_time_s": total / a.steps, "eigenpairs_per_solve": pairs // a.steps,
+                   "stages_s": {k: v / a.steps for k, v in stage_sum.items()},
+                   "kernels": {k: {"launches": v["launches"], "ms": v["ms"]} for k, v in kst.items()}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
