@@ -64,13 +64,21 @@ def ensure_tables():
 
 def run_engine(lib, wd, stats):
     """One solve in-process: chdir into the work directory and call nrg_main."""
-    args = [b"nrg", b"--stats", stats.encode(), b"--tables", os.path.join(PKG, "tables").encode()]
+    args = [b"nrg", b"--quiet", b"--stats", stats.encode(), b"--tables", os.path.join(PKG, "tables").encode()]
     argv = (ctypes.c_char_p * len(args))(*args)
     cwd = os.getcwd()
     os.chdir(wd)
+    # the engine logs to stdout; keep bench's stdout for the one JSON line
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
     try:
         rc = lib.nrg_main(len(args), argv)
     finally:
+        libc = ctypes.CDLL(None)
+        libc.fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
         os.chdir(cwd)
     if rc != 0:
         raise RuntimeError("nrg_main returned %d" % rc)

@@ -747,7 +747,7 @@ struct Engine {
                 nkept += kv.second.kept;
                 nall += kv.second.d;
             }
-            std::cout << "N=" << N << " subspaces=" << it.subs.size() << " Kept: " << nkept << " out of " << nall << std::endl;
+            if (!opt.quiet) std::cout << "N=" << N << " subspaces=" << it.subs.size() << " Kept: " << nkept << " out of " << nall << std::endl;
             // ---- recalculation with kept states (strategy=kept)
             st.old_op_arena = std::move(st.op_arena);
             st.op_arena.reset(new Arena(&gpu));
@@ -766,7 +766,7 @@ struct Engine {
         }
         GSenergy = totalenergy;
         scale_last = SCALE(Nmax);
-        std::cout << "Total energy: " << std::setprecision(16) << totalenergy << std::endl;
+        if (!opt.quiet) std::cout << "Total energy: " << std::setprecision(16) << totalenergy << std::endl;
         ftd.close();
         fcustom.close();
     }
@@ -1169,7 +1169,7 @@ struct Engine {
             double sum = 0;
             for (double x : wp) sum += x;
             for (double x : wq) sum += x;
-            std::cout << "Spectrum: " << name << " " << sp.kind << " sum=" << std::setprecision(12) << sum << std::endl;
+            if (!opt.quiet) std::cout << "Spectrum: " << name << " " << sp.kind << " sum=" << std::setprecision(12) << sum << std::endl;
             if (!P.broaden) continue;
             const double alpha = P.alpha, gamma = alpha / 4.0;
             const double omega0 = P.omega0 < 0.0 ? P.omega0_ratio * P.T : P.omega0;

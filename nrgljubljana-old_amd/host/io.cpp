@@ -72,7 +72,7 @@ void Params::load(const std::string &path) {
     std::string unused;
     for (auto &kv : raw)
         if (!used.count(kv.first)) unused += " " + kv.first;
-    if (!unused.empty()) std::cout << "Unused settings:" << unused << std::endl;
+    if (!unused.empty()) std::cerr << "Unused settings:" << unused << std::endl;
     discretization = discretization.substr(0, 1);
     // validateparameters (src/read-input.cc:6-36)
     if (keep <= 1) die("keep must be > 1");

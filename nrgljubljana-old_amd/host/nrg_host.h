@@ -135,6 +135,7 @@ struct Options {
     std::string stats;       // JSON with stage timings
     std::string tables_dir;
     int device = 0;
+    bool quiet = false;  // no per-iteration log lines
 };
 
 int run(const Options &opt);
