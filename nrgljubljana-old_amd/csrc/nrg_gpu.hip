@@ -176,46 +176,64 @@ __global__ __launch_bounds__(256) void k_assemble(const nrg_asm_block *blocks, c
 }
 
 // ---------------------------------------------------------------------------
-// eigensolver: one workgroup per matrix, matrix and workspace in HBM (L2)
+// eigensolver: one 1024-thread workgroup per matrix.
+//
+//  1. Householder tridiagonalisation A <- H_j A H_j (j = 0..n-3).  Step j's
+//     rank-2 update of the trailing matrix is fused with step j+1's
+//     matrix-vector product, so each step streams the trailing matrix once
+//     (one read + one write).  Threads own columns (coalesced rows).
+//  2. Q = H_0 ... H_{n-3} accumulated backwards into Z, again one fused pass
+//     (update of step j + reduction of step j-1) per reflector.
+//  3. A <- Q^T; implicit QL with Wilkinson shifts on (d, e).  Wave 0 computes
+//     the scalar recurrence of sweep t+1 while waves 1..15 apply the rotations
+//     of sweep t to the rows of A (double-buffered cosines/sines).
+//  4. Stable rank sort; rows gathered through Z.
+// Vectors live in LDS for n <= EIG_LDS_N, else in the global workspace.
 // ---------------------------------------------------------------------------
-constexpr int EIG_NT = 256;
+constexpr int EIG_NT = 1024;
+constexpr int EIG_NW = EIG_NT / 64;
+constexpr int EIG_LDS_N = 1344;  // 6 vectors x 1344 x 8 B = 63 KB dynamic LDS
+constexpr int EIG_NVEC = 12 + EIG_NW;  // vectors per matrix in the global workspace (+ row-group partials)
 
-struct EigWs {
-    double *Z;    // n*n  accumulated Q (row-major), later sort buffer
-    double *dg;   // n    diagonal
-    double *od;   // n    off-diagonal (od[i] couples i, i+1)
-    double *tau;  // n    Householder scalars
-    double *pv;   // n    vector temp
-    double *rc;   // n    rotation cosines of one QL sweep
-    double *rs;   // n    rotation sines
-    int *rank;    // n
-};
+__host__ __device__ inline size_t eig_ws_doubles(int n) { return (size_t)n * n + EIG_NVEC * (size_t)n; }
 
-__device__ inline EigWs eig_ws(double *base, int n) {
-    EigWs w;
-    w.Z = base;
-    double *v = base + (size_t)n * n;
-    w.dg = v;
-    w.od = v + n;
-    w.tau = v + 2 * n;
-    w.pv = v + 3 * n;
-    w.rc = v + 4 * n;
-    w.rs = v + 5 * n;
-    w.rank = reinterpret_cast<int *>(v + 6 * n);
-    return w;
+__device__ inline double eig_block_sum(double v, double *red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < EIG_NW; ++k) s += red[k];
+    return s;
 }
 
-__host__ __device__ inline size_t eig_ws_doubles(int n) { return (size_t)n * n + 7 * (size_t)n; }
+// Decomposition of a pass over an (mr x mc) block whose threads own columns:
+// waves take 64-column chunks, and when there are fewer chunks than waves the
+// rows are split into R contiguous groups so every wave keeps loads in flight.
+struct PassMap {
+    int ncc, R, rpg;
+};
+__device__ inline PassMap pass_map(int mr, int mc) {
+    PassMap p;
+    p.ncc = (mc + 63) / 64;
+    p.R = (p.ncc >= EIG_NW) ? 1 : EIG_NW / p.ncc;
+    if (p.R > mr) p.R = mr > 0 ? mr : 1;
+    p.rpg = (mr + p.R - 1) / p.R;
+    return p;
+}
 
-__global__ __launch_bounds__(EIG_NT) void k_eig(const nrg_eig_task *tasks, double *ws_base, const size_t *ws_off,
-                                                 int *status) {
-    __shared__ double red[EIG_NT / 64];
-    __shared__ int s_ctl[4];
-    __shared__ double s_val[2];
-    const nrg_eig_task tk = tasks[blockIdx.x];
+// IN_LDS selects at compile time where the work vectors live, so that the
+// LDS variant addresses them with ds_* instructions (a run-time choice would
+// turn every access into a flat access queued behind the global traffic).
+template <bool IN_LDS>
+__device__ __forceinline__ void eig_core(const nrg_eig_task &tk, double *ws_base, const size_t ws_off, int *status,
+                                         double *red, int *s_ctl, double *s_scal) {
+    extern __shared__ double lds[];
     const int n = tk.n;
-    double *A = tk.A;
-    double *W = tk.W;
+    double *__restrict__ A = tk.A;
+    double *__restrict__ W = tk.W;
     const int tid = threadIdx.x;
     if (n <= 0) return;
     if (n == 1) {
@@ -225,201 +243,442 @@ __global__ __launch_bounds__(EIG_NT) void k_eig(const nrg_eig_task *tasks, doubl
         }
         return;
     }
-    EigWs w = eig_ws(ws_base + ws_off[blockIdx.x], n);
+    double *base = ws_base + ws_off;
+    double *__restrict__ Z = base;
+    // global vectors: tau [0,n), pv [n,2n), spare [2n,3n), then 9 more if not in LDS
+    double *gv = base + (size_t)n * n;
+    constexpr bool in_lds = IN_LDS;
+    double *vecs = in_lds ? lds : gv + 3 * n;
+    double *va = vecs;          // reflector j (length m)
+    double *vb = vecs + n;      // reflector j+1 / j-1
+    double *wv = vecs + 2 * n;  // w = p - K v  /  accumulation row vector
+    double *dg = vecs + 3 * n;
+    double *od = vecs + 4 * n;
+    // QL rotation buffers reuse the reflector vectors (phases are disjoint)
+    double *rc0 = va, *rs0 = vb, *rc1 = wv, *rs1 = vecs + 5 * n;
+    double *tau = gv;           // n, global
+    double *pv = gv + n;        // p vector (global); rank array at the end
+    double *part = gv + 12 * (size_t)n;  // EIG_NW x n row-group partial sums
+    const int wave = tid >> 6, lane = tid & 63;
 
-    // ---- 1. Householder tridiagonalisation: A <- H_j A H_j, j = 0..n-3.
-    // The reflector of step j is stored in row j right of the diagonal.
+    // ---- 1. tridiagonalisation
+    bool p_ready = false;
     for (int j = 0; j < n - 2; ++j) {
-        const int m = n - j - 1;  // length of x = A[j][j+1..n-1]
-        double *xr = A + (size_t)j * n + j + 1;
-        double part = 0.0;
-        for (int k = 1 + tid; k < m; k += EIG_NT) part += xr[k] * xr[k];
-        const double sigma = block_sum<EIG_NT>(part, red);
-        const double x0 = xr[0];
-        if (sigma == 0.0) {
-            if (tid == 0) {
-                w.tau[j] = 0.0;
-                w.od[j] = x0;
+        const int m = n - j - 1;
+        double *xr = A + (size_t)j * n + j + 1;  // row j right of the diagonal (final)
+        double *A22 = A + (size_t)(j + 1) * n + (j + 1);
+        // reflector of step j (may have been prepared by the previous fused pass)
+        if (!p_ready) {
+            double psum = 0.0;
+            for (int k = 1 + tid; k < m; k += EIG_NT) psum += xr[k] * xr[k];
+            const double sigma = eig_block_sum(psum, red);
+            const double x0 = xr[0];
+            double t = 0.0;
+            if (sigma != 0.0) {
+                const double alpha = -copysign(sqrt(x0 * x0 + sigma), x0);
+                const double v0 = x0 - alpha;
+                t = 2.0 / (v0 * v0 + sigma);
+                for (int k = tid; k < m; k += EIG_NT) va[k] = (k == 0) ? v0 : xr[k];
+                if (tid == 0) od[j] = alpha;
+            } else if (tid == 0) {
+                od[j] = x0;
+            }
+            if (tid == 0) tau[j] = t;
+            __syncthreads();
+            if (t == 0.0) continue;  // nothing to do; next step reads row j+1 as is
+            if (tid == 0) xr[0] = va[0];
+            // p = tau * A22 * v (row groups -> partials -> sum)
+            {
+                const PassMap pm = pass_map(m, m);
+                for (int ch = (pm.ncc >= EIG_NW ? wave : wave % pm.ncc); ch < pm.ncc; ch += (pm.ncc >= EIG_NW ? EIG_NW : pm.ncc * EIG_NW)) {
+                    const int g = (pm.ncc >= EIG_NW) ? 0 : wave / pm.ncc;
+                    if (g >= pm.R) break;
+                    const int c = ch * 64 + lane;
+                    if (c >= m) continue;
+                    const int r0 = g * pm.rpg, r1 = min(m, r0 + pm.rpg);
+                    double acc = 0.0;
+                    int r = r0;
+                    for (; r + 8 <= r1; r += 8) {
+                        const double *q = A22 + (size_t)r * n + c;
+                        double a[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) a[u] = q[(size_t)u * n];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) acc += a[u] * va[r + u];
+                    }
+                    for (; r < r1; ++r) acc += A22[(size_t)r * n + c] * va[r];
+                    part[(size_t)g * n + c] = acc;
+                }
+                __syncthreads();
+                for (int c = tid; c < m; c += EIG_NT) {
+                    double acc = 0.0;
+                    for (int g = 0; g < pm.R; ++g) acc += part[(size_t)g * n + c];
+                    pv[c] = t * acc;
+                }
+                __syncthreads();
+            }
+        }
+        const double t = tau[j];
+        // K = tau/2 v.p ; w = p - K v
+        double pp = 0.0;
+        for (int k = tid; k < m; k += EIG_NT) pp += va[k] * pv[k];
+        const double K = 0.5 * t * eig_block_sum(pp, red);
+        for (int k = tid; k < m; k += EIG_NT) wv[k] = pv[k] - K * va[k];
+        __syncthreads();
+        // row 0 of A22 (row j+1 of A) first: it carries the next reflector
+        for (int c = tid; c < m; c += EIG_NT) A22[c] -= va[0] * wv[c] + wv[0] * va[c];
+        __syncthreads();
+        const bool has_next = (j + 1) < n - 2;
+        double tn = 0.0;
+        if (has_next) {
+            const int mn = m - 1;
+            double *xn = A22 + 1;  // row j+1, columns j+2..
+            double psum = 0.0;
+            for (int k = 1 + tid; k < mn; k += EIG_NT) psum += xn[k] * xn[k];
+            const double sigma = eig_block_sum(psum, red);
+            const double x0 = xn[0];
+            if (sigma != 0.0) {
+                const double alpha = -copysign(sqrt(x0 * x0 + sigma), x0);
+                const double v0 = x0 - alpha;
+                tn = 2.0 / (v0 * v0 + sigma);
+                for (int k = tid; k < mn; k += EIG_NT) vb[k] = (k == 0) ? v0 : xn[k];
+                if (tid == 0) od[j + 1] = alpha;
+            } else if (tid == 0) {
+                od[j + 1] = x0;
+            }
+            if (tid == 0) tau[j + 1] = tn;
+            __syncthreads();
+            if (tid == 0 && tn != 0.0) xn[0] = vb[0];
+        }
+        // fused pass over rows/cols 1..m-1 of A22: update, and p' = tn * A33 v'
+        const bool fuse = has_next && tn != 0.0;
+        {
+            const int mm = m - 1;
+            const PassMap pm = pass_map(mm, mm);
+            for (int ch = (pm.ncc >= EIG_NW ? wave : wave % pm.ncc); ch < pm.ncc; ch += (pm.ncc >= EIG_NW ? EIG_NW : pm.ncc * EIG_NW)) {
+                const int g = (pm.ncc >= EIG_NW) ? 0 : wave / pm.ncc;
+                if (g >= pm.R) break;
+                const int c = 1 + ch * 64 + lane;
+                if (c >= m) continue;
+                const double wc = wv[c], vc = va[c];
+                const int r0 = 1 + g * pm.rpg, r1 = min(m, r0 + pm.rpg);
+                double acc = 0.0;
+                int r = r0;
+                for (; r + 8 <= r1; r += 8) {
+                    double *q = A22 + (size_t)r * n + c;
+                    double a[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) a[u] = q[(size_t)u * n];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        a[u] -= va[r + u] * wc + wv[r + u] * vc;
+                        q[(size_t)u * n] = a[u];
+                        acc += a[u] * vb[r + u - 1];
+                    }
+                }
+                for (; r < r1; ++r) {
+                    double *q = A22 + (size_t)r * n + c;
+                    const double a0 = q[0] - (va[r] * wc + wv[r] * vc);
+                    q[0] = a0;
+                    acc += a0 * vb[r - 1];
+                }
+                if (fuse) part[(size_t)g * n + (c - 1)] = acc;
             }
             __syncthreads();
-            continue;
-        }
-        const double alpha = -copysign(sqrt(x0 * x0 + sigma), x0);
-        const double v0 = x0 - alpha;
-        const double tau = 2.0 / (v0 * v0 + sigma);
-        __syncthreads();
-        if (tid == 0) {
-            xr[0] = v0;
-            w.tau[j] = tau;
-            w.od[j] = alpha;
+            if (fuse) {
+                for (int c = tid; c < mm; c += EIG_NT) {
+                    double acc = 0.0;
+                    for (int g = 0; g < pm.R; ++g) acc += part[(size_t)g * n + c];
+                    pv[c] = tn * acc;
+                }
+            }
         }
         __syncthreads();
-        // p = tau * A22 v (column access: coalesced across threads)
-        double *A22 = A + (size_t)(j + 1) * n + (j + 1);
-        for (int i = tid; i < m; i += EIG_NT) {
-            double s = 0.0;
-            for (int k = 0; k < m; ++k) s += A22[(size_t)k * n + i] * xr[k];
-            w.pv[i] = tau * s;
+        if (fuse) {
+            // the next reflector becomes current
+            for (int k = tid; k < m - 1; k += EIG_NT) va[k] = vb[k];
+            __syncthreads();
         }
-        __syncthreads();
-        double pp = 0.0;
-        for (int i = tid; i < m; i += EIG_NT) pp += xr[i] * w.pv[i];
-        const double K = 0.5 * tau * block_sum<EIG_NT>(pp, red);
-        for (int i = tid; i < m; i += EIG_NT) w.pv[i] -= K * xr[i];
-        __syncthreads();
-        // A22 -= v w^T + w v^T
-        for (int r = 0; r < m; ++r) {
-            const double vr = xr[r], wr = w.pv[r];
-            double *row = A22 + (size_t)r * n;
-            for (int c = tid; c < m; c += EIG_NT) row[c] -= vr * w.pv[c] + wr * xr[c];
-        }
-        __syncthreads();
+        p_ready = fuse;
     }
     if (tid == 0) {
-        w.od[n - 2] = A[(size_t)(n - 2) * n + n - 1];
-        w.od[n - 1] = 0.0;
+        od[n - 2] = A[(size_t)(n - 2) * n + n - 1];
+        od[n - 1] = 0.0;
     }
-    for (int i = tid; i < n; i += EIG_NT) w.dg[i] = A[(size_t)i * n + i];
-    // ---- 2. Q = H_0 H_1 ... H_{n-3}, backward accumulation into Z.
+    for (int i = tid; i < n; i += EIG_NT) dg[i] = A[(size_t)i * n + i];
+    // ---- 2. Q = H_0 ... H_{n-3}: Z = I, then backwards, one fused pass per step
     for (size_t e = tid; e < (size_t)n * n; e += EIG_NT) {
         const size_t r = e / n, cc = e - r * n;
-        w.Z[e] = (r == cc) ? 1.0 : 0.0;
+        Z[e] = (r == cc) ? 1.0 : 0.0;
     }
     __syncthreads();
-    for (int j = n - 3; j >= 0; --j) {
-        const double tau = w.tau[j];
-        if (tau == 0.0) continue;
-        const int m = n - j - 1;
-        const double *v = A + (size_t)j * n + j + 1;
-        double *Zb = w.Z + (size_t)(j + 1) * n + (j + 1);
-        for (int c = tid; c < m; c += EIG_NT) {
-            double s = 0.0;
-            for (int r = 0; r < m; ++r) s += v[r] * Zb[(size_t)r * n + c];
-            w.pv[c] = tau * s;
+    if (n >= 3) {
+        // wt_j for the first processed reflector j = n-3 on an identity block: wt = v
+        int j = n - 3;
+        {
+            const int m = n - j - 1;
+            const double *v = A + (size_t)j * n + j + 1;
+            for (int k = tid; k < m; k += EIG_NT) {
+                va[k] = v[k];
+                wv[k] = v[k];
+            }
         }
         __syncthreads();
-        for (int r = 0; r < m; ++r) {
-            const double vr = v[r];
-            double *row = Zb + (size_t)r * n;
-            for (int c = tid; c < m; c += EIG_NT) row[c] -= vr * w.pv[c];
+        for (; j >= 0; --j) {
+            const int m = n - j - 1;
+            const double t = tau[j];
+            const bool has_prev = j >= 1;
+            const double tp = has_prev ? tau[j - 1] : 0.0;
+            if (has_prev) {
+                const double *v = A + (size_t)(j - 1) * n + j;  // reflector j-1, length m+1
+                for (int k = tid; k <= m; k += EIG_NT) vb[k] = v[k];
+            }
+            __syncthreads();
+            double *Zb = Z + (size_t)(j + 1) * n + (j + 1);
+            // thread column c: Z[j+1+r][j+1+c] -= t*va[r]*wv[c]; acc += vb[1+r]*Znew
+            const PassMap pm = pass_map(m, m);
+            for (int ch = (pm.ncc >= EIG_NW ? wave : wave % pm.ncc); ch < pm.ncc; ch += (pm.ncc >= EIG_NW ? EIG_NW : pm.ncc * EIG_NW)) {
+                const int g = (pm.ncc >= EIG_NW) ? 0 : wave / pm.ncc;
+                if (g >= pm.R) break;
+                const int c = ch * 64 + lane;
+                if (c >= m) continue;
+                const double tw = t * wv[c];
+                const int r0 = g * pm.rpg, r1 = min(m, r0 + pm.rpg);
+                double acc = 0.0;
+                int r = r0;
+                for (; r + 8 <= r1; r += 8) {
+                    double *q = Zb + (size_t)r * n + c;
+                    double z[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) z[u] = q[(size_t)u * n];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        z[u] -= va[r + u] * tw;
+                        if (t != 0.0) q[(size_t)u * n] = z[u];
+                        acc += vb[1 + r + u] * z[u];
+                    }
+                }
+                for (; r < r1; ++r) {
+                    double *q = Zb + (size_t)r * n + c;
+                    const double z0 = q[0] - va[r] * tw;
+                    if (t != 0.0) q[0] = z0;
+                    acc += vb[1 + r] * z0;
+                }
+                if (has_prev) part[(size_t)g * n + c] = acc;
+            }
+            __syncthreads();
+            if (has_prev) {
+                for (int c = tid; c < m; c += EIG_NT) {
+                    double acc = 0.0;
+                    for (int g = 0; g < pm.R; ++g) acc += part[(size_t)g * n + c];
+                    pv[1 + c] = acc;
+                }
+            }
+            __syncthreads();
+            if (has_prev) {
+                // next step: reflector j-1 becomes current, wt from this pass
+                for (int k = tid; k <= m; k += EIG_NT) {
+                    va[k] = vb[k];
+                    wv[k] = (k == 0) ? vb[0] : pv[k];
+                }
+                __syncthreads();
+            }
+            (void)tp;
         }
-        __syncthreads();
     }
-    // ---- 3. A <- Q^T (row i of A = column i of Q)
+    // ---- 3. A <- Z^T (row i of A = column i of Q)
     for (size_t e = tid; e < (size_t)n * n; e += EIG_NT) {
         const size_t r = e / n, cc = e - r * n;
-        A[r * n + cc] = w.Z[cc * n + r];
+        A[cc * n + r] = Z[e];
     }
     __syncthreads();
-    // ---- 4. implicit QL on (dg, od); rotations act on rows (i, i+1) of A.
+    // ---- 4. implicit QL; wave 0 computes sweep t+1 while waves 1.. apply sweep t
+    // s_ctl: [0] l, [1] iteration count, [2+b] lo of buffer b, [4+b] hi of buffer b, [6] done
     if (tid == 0) {
-        s_ctl[0] = 0;  // l
-        s_ctl[3] = 0;  // iteration counter for current l
+        s_ctl[0] = 0;
+        s_ctl[1] = 0;
+        s_ctl[2] = s_ctl[3] = -1;
+        s_ctl[4] = s_ctl[5] = -2;
+        s_ctl[6] = 0;
     }
     __syncthreads();
     const double eps = 2.220446049250313e-16;
-    for (;;) {
-        if (tid == 0) {
+    for (int round = 0;; ++round) {
+        const int b = round & 1;
+        const int done_before = s_ctl[6];
+        __syncthreads();  // everyone has read the flag before thread 0 may set it
+        if (wave == 0 && !done_before) {
+            // the whole wave looks for the deflation point; lane 0 runs the sweep
+            double *rc = b ? rc1 : rc0;
+            double *rs = b ? rs1 : rs0;
             int l = s_ctl[0];
-            int lo = -1, hi = -1, done = 0;
+            int iter = s_ctl[1];
+            int m = l, done = 0;
             for (;;) {
                 if (l >= n) {
                     done = 1;
                     break;
                 }
-                int m;
-                for (m = l; m < n - 1; ++m) {
-                    const double dd = fabs(w.dg[m]) + fabs(w.dg[m + 1]);
-                    if (fabs(w.od[m]) <= eps * dd) break;
+                m = n - 1;
+                for (int base = l; base < n - 1; base += 64) {
+                    const int idx = base + lane;
+                    bool small = false;
+                    if (idx < n - 1) {
+                        const double dd = fabs(dg[idx]) + fabs(dg[idx + 1]);
+                        small = fabs(od[idx]) <= eps * dd;
+                    }
+                    const unsigned long long bal = __ballot(small);
+                    if (bal) {
+                        m = base + __ffsll((long long)bal) - 1;
+                        break;
+                    }
                 }
                 if (m == l) {
                     ++l;
-                    s_ctl[3] = 0;
+                    iter = 0;
                     continue;
                 }
-                if (++s_ctl[3] > 60) {
-                    atomicExch(status, NRG_ERR_NOCONV);
+                if (++iter > 60) {
+                    if (lane == 0) atomicExch(status, NRG_ERR_NOCONV);
                     done = 1;
-                    break;
-                }
-                double g = (w.dg[l + 1] - w.dg[l]) / (2.0 * w.od[l]);
-                double r = hypot(g, 1.0);
-                g = w.dg[m] - w.dg[l] + w.od[l] / (g + copysign(r, g));
-                double s = 1.0, c = 1.0, p = 0.0;
-                int i;
-                bool early = false;
-                for (i = m - 1; i >= l; --i) {
-                    double f = s * w.od[i];
-                    const double b = c * w.od[i];
-                    r = hypot(f, g);
-                    w.od[i + 1] = r;
-                    if (r == 0.0) {
-                        w.dg[i + 1] -= p;
-                        w.od[m] = 0.0;
-                        early = true;
-                        break;
-                    }
-                    s = f / r;
-                    c = g / r;
-                    g = w.dg[i + 1] - p;
-                    r = (w.dg[i] - g) * s + 2.0 * c * b;
-                    p = s * r;
-                    w.dg[i + 1] = g + p;
-                    g = c * r - b;
-                    w.rc[i] = c;
-                    w.rs[i] = s;
-                }
-                hi = m - 1;
-                lo = early ? i + 1 : l;
-                if (!early) {
-                    w.dg[l] -= p;
-                    w.od[l] = g;
-                    w.od[m] = 0.0;
                 }
                 break;
             }
-            s_ctl[0] = l;
-            s_ctl[1] = done ? -1 : lo;
-            s_ctl[2] = hi;
-        }
-        __syncthreads();
-        const int lo = s_ctl[1], hi = s_ctl[2];
-        if (lo < 0) break;
-        if (hi >= lo) {
-            for (int k = tid; k < n; k += EIG_NT) {
-                double up = A[(size_t)(hi + 1) * n + k];
-                for (int i = hi; i >= lo; --i) {
-                    const double cur = A[(size_t)i * n + k];
-                    const double c = w.rc[i], s = w.rs[i];
-                    A[(size_t)(i + 1) * n + k] = s * cur + c * up;
-                    up = c * cur - s * up;
+            if (lane == 0) {
+                int lo = -1, hi = -2;
+                if (!done) {
+                    double g = (dg[l + 1] - dg[l]) / (2.0 * od[l]);
+                    double r = sqrt(g * g + 1.0);
+                    g = dg[m] - dg[l] + od[l] / (g + copysign(r, g));
+                    double s = 1.0, c = 1.0, p = 0.0;
+                    int i;
+                    bool early = false;
+                    double odi = od[m - 1], dgi = dg[m - 1], dgi1 = dg[m];
+                    for (i = m - 1; i >= l; --i) {
+                        const double f = s * odi;
+                        const double bb = c * odi;
+                        // prefetch the next iteration's operands off the critical path
+                        const double odn = (i > l) ? od[i - 1] : 0.0;
+                        const double dgn = (i > l) ? dg[i - 1] : 0.0;
+                        const double x = f * f + g * g;
+                        if (x == 0.0) {
+                            od[i + 1] = 0.0;
+                            dg[i + 1] = dgi1 - p;
+                            od[m] = 0.0;
+                            early = true;
+                            break;
+                        }
+                        // 1/sqrt with one Newton step: |c^2 + s^2 - 1| ~ 1 ulp
+                        double y = __builtin_amdgcn_rsq(x);
+                        y = y * (1.5 - 0.5 * x * y * y);
+                        r = x * y;
+                        od[i + 1] = r;
+                        s = f * y;
+                        c = g * y;
+                        g = dgi1 - p;
+                        r = (dgi - g) * s + 2.0 * c * bb;
+                        p = s * r;
+                        dg[i + 1] = g + p;
+                        g = c * r - bb;
+                        rc[i] = c;
+                        rs[i] = s;
+                        dgi1 = dgi;
+                        dgi = dgn;
+                        odi = odn;
+                    }
+                    hi = m - 1;
+                    lo = early ? i + 1 : l;
+                    if (!early) {
+                        dg[l] -= p;
+                        od[l] = g;
+                        od[m] = 0.0;
+                    }
                 }
-                A[(size_t)lo * n + k] = up;
+                s_ctl[0] = l;
+                s_ctl[1] = iter;
+                s_ctl[2 + b] = lo;
+                s_ctl[4 + b] = hi;
+                if (done) {
+                    s_ctl[6] = 1;
+                    s_ctl[2 + b] = -1;
+                    s_ctl[4 + b] = -2;
+                }
+            }
+        }
+        if (tid >= 64) {
+            // apply the sweep computed in the previous round
+            const int pb = b ^ 1;
+            const int lo = s_ctl[2 + pb], hi = s_ctl[4 + pb];
+            if (round > 0 && hi >= lo && lo >= 0) {
+                const double *rc = pb ? rc1 : rc0;
+                const double *rs = pb ? rs1 : rs0;
+                for (int k = tid - 64; k < n; k += EIG_NT - 64) {
+                    double *col = A + k;
+                    double up = col[(size_t)(hi + 1) * n];
+                    int i = hi;
+                    // 8 rows at a time: loads of the block issued before its stores
+                    for (; i - 7 >= lo; i -= 8) {
+                        double cur[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) cur[u] = col[(size_t)(i - u) * n];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const double c = rc[i - u], s = rs[i - u];
+                            col[(size_t)(i - u + 1) * n] = s * cur[u] + c * up;
+                            up = c * cur[u] - s * up;
+                        }
+                    }
+                    for (; i >= lo; --i) {
+                        const double cur = col[(size_t)i * n];
+                        const double c = rc[i], s = rs[i];
+                        col[(size_t)(i + 1) * n] = s * cur + c * up;
+                        up = c * cur - s * up;
+                    }
+                    col[(size_t)lo * n] = up;
+                }
             }
         }
         __syncthreads();
+        if (done_before) break;  // the last computed sweep has now been applied
     }
-    // ---- 5. sort ascending (stable rank), rows gathered through Z.
+    // ---- 5. sort ascending (stable rank), rows gathered through Z
+    int *rank = reinterpret_cast<int *>(pv);
     for (int i = tid; i < n; i += EIG_NT) {
-        const double di = w.dg[i];
+        const double di = dg[i];
         int rk = 0;
-        for (int j = 0; j < n; ++j) {
-            const double dj = w.dg[j];
-            rk += (dj < di) || (dj == di && j < i);
+        for (int jj = 0; jj < n; ++jj) {
+            const double dj = dg[jj];
+            rk += (dj < di) || (dj == di && jj < i);
         }
-        w.rank[i] = rk;
+        rank[i] = rk;
+        W[rk] = di;
     }
     __syncthreads();
-    for (int i = 0; i < n; ++i) {
-        const int rk = w.rank[i];
-        const double *src = A + (size_t)i * n;
-        double *dst = w.Z + (size_t)rk * n;
-        for (int k = tid; k < n; k += EIG_NT) dst[k] = src[k];
-        if (tid == 0) W[rk] = w.dg[i];
+    for (size_t e = tid; e < (size_t)n * n; e += EIG_NT) {
+        const size_t i = e / n, k = e - i * n;
+        Z[(size_t)rank[i] * n + k] = A[e];
     }
     __syncthreads();
-    for (size_t e = tid; e < (size_t)n * n; e += EIG_NT) A[e] = w.Z[e];
-    (void)s_val;
+    for (size_t e = tid; e < (size_t)n * n; e += EIG_NT) A[e] = Z[e];
+    (void)s_scal;
+}
+
+__global__ __launch_bounds__(EIG_NT) void k_eig_lds(const nrg_eig_task *tasks, double *ws_base, const size_t *ws_off,
+                                                     int *status) {
+    __shared__ double red[EIG_NW];
+    __shared__ double s_scal[4];
+    __shared__ int s_ctl[8];
+    const nrg_eig_task tk = tasks[blockIdx.x];
+    eig_core<true>(tk, ws_base, ws_off[blockIdx.x], status, red, s_ctl, s_scal);
+}
+
+__global__ __launch_bounds__(EIG_NT) void k_eig_glb(const nrg_eig_task *tasks, double *ws_base, const size_t *ws_off,
+                                                     int *status) {
+    __shared__ double red[EIG_NW];
+    __shared__ double s_scal[4];
+    __shared__ int s_ctl[8];
+    const nrg_eig_task tk = tasks[blockIdx.x];
+    eig_core<false>(tk, ws_base, ws_off[blockIdx.x], status, red, s_ctl, s_scal);
 }
 
 // ---------------------------------------------------------------------------
@@ -718,8 +977,13 @@ nrg_status nrg_assemble_batch(nrg_ctx *c, const nrg_asm_block *blocks, int nbloc
     return NRG_OK;
 }
 
-nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks, int ntasks) {
+nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, int ntasks) {
     if (ntasks <= 0) return NRG_OK;
+    // largest first: better load balance, and the LDS-sized tail is contiguous
+    std::vector<nrg_eig_task> sorted(tasks_in, tasks_in + ntasks);
+    std::stable_sort(sorted.begin(), sorted.end(),
+                     [](const nrg_eig_task &a, const nrg_eig_task &b) { return a.n > b.n; });
+    const nrg_eig_task *tasks = sorted.data();
     std::vector<size_t> off(ntasks);
     size_t tot = 0;
     for (int i = 0; i < ntasks; ++i) {
@@ -749,10 +1013,23 @@ nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks, int ntasks) {
         eflops += 10.0 / 3.0 * n * n * n;  // tridiagonalisation + back-transform
         ebytes += 8.0 / 3.0 * n * n * n;   // trailing matrix read once per column
     }
+    // tasks arrive largest first; the ones that fit the LDS variant form the tail
+    int split = ntasks;
+    while (split > 0 && tasks[split - 1].n <= EIG_LDS_N) --split;
+    for (int i = 0; i < split; ++i)
+        if (tasks[i].n <= EIG_LDS_N) return fail(c, NRG_ERR_ARG, "diag: tasks must be sorted by decreasing size");
+    int nl = 0;
+    for (int i = split; i < ntasks; ++i) nl = std::max(nl, tasks[i].n);
+    const size_t lds = 6 * (size_t)nl * sizeof(double);
     {
     LaunchTimer lt(K_EIG, c->stream, eflops, ebytes);
-    hipLaunchKernelGGL(k_eig, dim3(ntasks), dim3(EIG_NT), 0, c->stream, (const nrg_eig_task *)d, c->ews,
-                       (const size_t *)((char *)d + tb), c->dstatus);
+    if (split > 0)
+        hipLaunchKernelGGL(k_eig_glb, dim3(split), dim3(EIG_NT), 0, c->stream, (const nrg_eig_task *)d, c->ews,
+                           (const size_t *)((char *)d + tb), c->dstatus);
+    if (split < ntasks)
+        hipLaunchKernelGGL(k_eig_lds, dim3(ntasks - split), dim3(EIG_NT), lds, c->stream,
+                           (const nrg_eig_task *)d + split, c->ews, (const size_t *)((char *)d + tb) + split,
+                           c->dstatus);
     }
     HIPCHK(c, hipGetLastError());
     int st = 0;
