@@ -1,0 +1,1021 @@
+// nrg_eig.hip - batched FP64 symmetric eigensolver (replaces the dsyev call
+// of diagonalise(), src/diag.h:54-89 and 297-353).
+//
+// All matrices of one NRG iteration are solved together, stage by stage:
+//   k_eig_tri    Householder tridiagonalisation, one 1024-thread workgroup per
+//                matrix; step j's rank-2 update is fused with step j+1's
+//                matrix-vector product (one pass over the trailing matrix per
+//                step).  Reflectors stay in the rows of A.
+//   k_dc_*       Cuppen divide and conquer on the tridiagonal matrix: the
+//                tridiagonal is cut into leaves of 1 or 2 rows, then merged
+//                level by level for all matrices at once.  A merge sorts the
+//                poles, deflates (small z components, close poles by a Givens
+//                rotation), solves every secular equation in parallel
+//                (bracketed two-pole rational iteration in the coordinates of
+//                the nearer pole), forms Gu-Eisenstat z-hat from the computed
+//                roots (Loewner), normalises the eigenvectors of D + rho z z^T
+//                and multiplies them into the merged block with the grouped
+//                GEMM.  Eigenvectors of the tridiagonal are kept column-major
+//                (one contiguous row per eigenvector).
+//   k_eig_bt     back-transformation: one wavefront per eigenvector applies
+//                the reflectors H_{n-3} .. H_0 with the vector held in
+//                registers, and writes it as row rank(c) of the result.
+// Output convention as diagonalise(): eigenvalues ascending in W, row r of A
+// = eigenvector r.
+#include "nrg_internal.h"
+
+constexpr int TRI_NT = 1024;
+constexpr int TRI_NW = TRI_NT / 64;
+constexpr int TRI_LDS_N = 1344;  // 5 vectors x 1344 x 8 B = 53 KB dynamic LDS
+constexpr int DEFL_NT = 256;
+constexpr int DEFL_LDS_K = 2048;  // 2 x K doubles + 2 x K ints + K doubles = 32 B x K
+constexpr int SEC_NT = 64;
+constexpr double kEps = 2.220446049250313e-16;
+
+// Per-matrix workspace: Vc, S1, S2 (n x n each), then vectors of n doubles.
+//  Vc  eigenvectors of the tridiagonal, column-major (row c = eigenvector c)
+//  S1  gathered / rotated eigenvector block of a merge; result of the
+//      back-transformation
+//  S2  Delta_ji = d_i - lambda_j of a merge, overwritten by the normalised
+//      eigenvectors of D + rho z z^T
+enum VecSlot {
+    V_TAU = 0,  // Householder scalars
+    V_PV,       // p = tau A v
+    V_VA,       // tridiagonalisation vectors (global variant)
+    V_VB,
+    V_WV,
+    V_DGT,
+    V_ODT,
+    V_DG,    // diagonal of T (after tridiagonalisation)
+    V_OD,    // off-diagonal of T
+    V_D,     // current eigenvalues (per position)
+    V_BETA,  // coupling cut at a merge boundary, stored at index mid
+    V_DD,    // sorted non-deflated poles of a merge (at a..a+K')
+    V_ZZ,    // their z components
+    V_ZH,    // z-hat
+    V_TS,    // secular root offsets
+    V_ORG,   // int: origin pole of each root
+    V_META,  // int: [a] = K', [a+1] = number of rotations
+    V_RHO,   // rho of the merge at [a]
+    V_RANK,  // int: final rank of each eigenvalue
+    V_RP,    // int: rotation pairs (sorted indices)
+    V_RQ,
+    V_RC,
+    V_RS,
+    V_SDS,   // deflation arrays (global variant)
+    V_SZS,
+    V_PERM,
+    V_POS,
+    V_PART,  // TRI_NW vectors of row-group partial sums
+    V_COUNT = V_PART + TRI_NW
+};
+
+__host__ __device__ inline size_t eig_ws_doubles(int n) { return 3 * (size_t)n * n + (size_t)V_COUNT * n; }
+__device__ inline double *vslot(double *ws, int n, int s) { return ws + 3 * (size_t)n * n + (size_t)s * n; }
+__device__ inline int *islot(double *ws, int n, int s) { return reinterpret_cast<int *>(vslot(ws, n, s)); }
+
+struct EigMat {
+    double *A;
+    double *W;
+    double *ws;
+    int n;
+    int pad;
+};
+
+struct DcMerge {
+    double *ws;
+    int n, a, mid, b;
+    int gemm;  // index of this merge's output descriptor
+    int pad;
+};
+
+struct DcChunk {
+    int merge;
+    int j0;
+};
+
+struct BtItem {
+    int mat;
+    int c0;
+};
+
+__device__ inline double tri_block_sum(double v, double *red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < TRI_NW; ++k) s += red[k];
+    return s;
+}
+
+// Decomposition of a pass over an (mr x mc) block whose threads own columns:
+// waves take 64-column chunks, and when there are fewer chunks than waves the
+// rows are split into R contiguous groups so every wave keeps loads in flight.
+struct PassMap {
+    int ncc, R, rpg;
+};
+__device__ inline PassMap pass_map(int mr, int mc) {
+    PassMap p;
+    p.ncc = (mc + 63) / 64;
+    p.R = (p.ncc >= TRI_NW) ? 1 : TRI_NW / p.ncc;
+    if (p.R > mr) p.R = mr > 0 ? mr : 1;
+    p.rpg = (mr + p.R - 1) / p.R;
+    return p;
+}
+
+// ---------------------------------------------------------------------------
+// 1. tridiagonalisation: A <- H_j A H_j, j = 0..n-3, H_j = I - tau_j v_j v_j^T
+//    acting on rows/columns j+1..n-1.  v_j is left in row j of A (columns
+//    j+1..n-1, first element included), tau_j in V_TAU, T in V_DG / V_OD.
+// IN_LDS selects at compile time where the work vectors live, so that the
+// LDS variant addresses them with ds_* instructions.
+// ---------------------------------------------------------------------------
+template <bool IN_LDS>
+__device__ __forceinline__ void tri_core(const EigMat &mt, double *red) {
+    extern __shared__ double lds[];
+    const int n = mt.n;
+    double *__restrict__ A = mt.A;
+    const int tid = threadIdx.x;
+    if (n <= 0) return;
+    if (n == 1) {
+        if (tid == 0) {
+            mt.W[0] = A[0];
+            A[0] = 1.0;
+        }
+        return;
+    }
+    double *ws = mt.ws;
+    double *vecs = IN_LDS ? lds : vslot(ws, n, V_VA);
+    double *va = vecs;          // reflector j (length m)
+    double *vb = vecs + n;      // reflector j+1
+    double *wv = vecs + 2 * n;  // w = p - K v
+    double *dg = vecs + 3 * n;
+    double *od = vecs + 4 * n;
+    double *tau = vslot(ws, n, V_TAU);
+    double *pv = vslot(ws, n, V_PV);
+    double *part = vslot(ws, n, V_PART);
+    const int wave = tid >> 6, lane = tid & 63;
+
+    bool p_ready = false;
+    for (int j = 0; j < n - 2; ++j) {
+        const int m = n - j - 1;
+        double *xr = A + (size_t)j * n + j + 1;  // row j right of the diagonal (final)
+        double *A22 = A + (size_t)(j + 1) * n + (j + 1);
+        // reflector of step j (may have been prepared by the previous fused pass)
+        if (!p_ready) {
+            double psum = 0.0;
+            for (int k = 1 + tid; k < m; k += TRI_NT) psum += xr[k] * xr[k];
+            const double sigma = tri_block_sum(psum, red);
+            const double x0 = xr[0];
+            double t = 0.0;
+            if (sigma != 0.0) {
+                const double alpha = -copysign(sqrt(x0 * x0 + sigma), x0);
+                const double v0 = x0 - alpha;
+                t = 2.0 / (v0 * v0 + sigma);
+                for (int k = tid; k < m; k += TRI_NT) va[k] = (k == 0) ? v0 : xr[k];
+                if (tid == 0) od[j] = alpha;
+            } else if (tid == 0) {
+                od[j] = x0;
+            }
+            if (tid == 0) tau[j] = t;
+            __syncthreads();
+            if (t == 0.0) continue;  // nothing to do; next step reads row j+1 as is
+            if (tid == 0) xr[0] = va[0];
+            // p = tau * A22 * v (row groups -> partials -> sum)
+            {
+                const PassMap pm = pass_map(m, m);
+                for (int ch = (pm.ncc >= TRI_NW ? wave : wave % pm.ncc); ch < pm.ncc;
+                     ch += (pm.ncc >= TRI_NW ? TRI_NW : pm.ncc * TRI_NW)) {
+                    const int g = (pm.ncc >= TRI_NW) ? 0 : wave / pm.ncc;
+                    if (g >= pm.R) break;
+                    const int c = ch * 64 + lane;
+                    if (c >= m) continue;
+                    const int r0 = g * pm.rpg, r1 = min(m, r0 + pm.rpg);
+                    double acc = 0.0;
+                    int r = r0;
+                    for (; r + 8 <= r1; r += 8) {
+                        const double *q = A22 + (size_t)r * n + c;
+                        double a[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) a[u] = q[(size_t)u * n];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) acc += a[u] * va[r + u];
+                    }
+                    for (; r < r1; ++r) acc += A22[(size_t)r * n + c] * va[r];
+                    part[(size_t)g * n + c] = acc;
+                }
+                __syncthreads();
+                for (int c = tid; c < m; c += TRI_NT) {
+                    double acc = 0.0;
+                    for (int g = 0; g < pm.R; ++g) acc += part[(size_t)g * n + c];
+                    pv[c] = t * acc;
+                }
+                __syncthreads();
+            }
+        }
+        const double t = tau[j];
+        // K = tau/2 v.p ; w = p - K v
+        double pp = 0.0;
+        for (int k = tid; k < m; k += TRI_NT) pp += va[k] * pv[k];
+        const double K = 0.5 * t * tri_block_sum(pp, red);
+        for (int k = tid; k < m; k += TRI_NT) wv[k] = pv[k] - K * va[k];
+        __syncthreads();
+        // row 0 of A22 (row j+1 of A) first: it carries the next reflector
+        for (int c = tid; c < m; c += TRI_NT) A22[c] -= va[0] * wv[c] + wv[0] * va[c];
+        __syncthreads();
+        const bool has_next = (j + 1) < n - 2;
+        double tn = 0.0;
+        if (has_next) {
+            const int mn = m - 1;
+            double *xn = A22 + 1;  // row j+1, columns j+2..
+            double psum = 0.0;
+            for (int k = 1 + tid; k < mn; k += TRI_NT) psum += xn[k] * xn[k];
+            const double sigma = tri_block_sum(psum, red);
+            const double x0 = xn[0];
+            if (sigma != 0.0) {
+                const double alpha = -copysign(sqrt(x0 * x0 + sigma), x0);
+                const double v0 = x0 - alpha;
+                tn = 2.0 / (v0 * v0 + sigma);
+                for (int k = tid; k < mn; k += TRI_NT) vb[k] = (k == 0) ? v0 : xn[k];
+                if (tid == 0) od[j + 1] = alpha;
+            } else if (tid == 0) {
+                od[j + 1] = x0;
+            }
+            if (tid == 0) tau[j + 1] = tn;
+            __syncthreads();
+            if (tid == 0 && tn != 0.0) xn[0] = vb[0];
+        }
+        // fused pass over rows/cols 1..m-1 of A22: update, and p' = tn * A33 v'
+        const bool fuse = has_next && tn != 0.0;
+        {
+            const int mm = m - 1;
+            const PassMap pm = pass_map(mm, mm);
+            for (int ch = (pm.ncc >= TRI_NW ? wave : wave % pm.ncc); ch < pm.ncc;
+                 ch += (pm.ncc >= TRI_NW ? TRI_NW : pm.ncc * TRI_NW)) {
+                const int g = (pm.ncc >= TRI_NW) ? 0 : wave / pm.ncc;
+                if (g >= pm.R) break;
+                const int c = 1 + ch * 64 + lane;
+                if (c >= m) continue;
+                const double wc = wv[c], vc = va[c];
+                const int r0 = 1 + g * pm.rpg, r1 = min(m, r0 + pm.rpg);
+                double acc = 0.0;
+                int r = r0;
+                for (; r + 8 <= r1; r += 8) {
+                    double *q = A22 + (size_t)r * n + c;
+                    double a[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) a[u] = q[(size_t)u * n];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        a[u] -= va[r + u] * wc + wv[r + u] * vc;
+                        q[(size_t)u * n] = a[u];
+                        acc += a[u] * vb[r + u - 1];
+                    }
+                }
+                for (; r < r1; ++r) {
+                    double *q = A22 + (size_t)r * n + c;
+                    const double a0 = q[0] - (va[r] * wc + wv[r] * vc);
+                    q[0] = a0;
+                    acc += a0 * vb[r - 1];
+                }
+                if (fuse) part[(size_t)g * n + (c - 1)] = acc;
+            }
+            __syncthreads();
+            if (fuse) {
+                for (int c = tid; c < mm; c += TRI_NT) {
+                    double acc = 0.0;
+                    for (int g = 0; g < pm.R; ++g) acc += part[(size_t)g * n + c];
+                    pv[c] = tn * acc;
+                }
+            }
+        }
+        __syncthreads();
+        if (fuse) {
+            // the next reflector becomes current
+            for (int k = tid; k < m - 1; k += TRI_NT) va[k] = vb[k];
+            __syncthreads();
+        }
+        p_ready = fuse;
+    }
+    if (tid == 0) {
+        od[n - 2] = A[(size_t)(n - 2) * n + n - 1];
+        od[n - 1] = 0.0;
+    }
+    for (int i = tid; i < n; i += TRI_NT) dg[i] = A[(size_t)i * n + i];
+    __syncthreads();
+    double *gdg = vslot(ws, n, V_DG), *god = vslot(ws, n, V_OD);
+    for (int i = tid; i < n; i += TRI_NT) {
+        gdg[i] = dg[i];
+        god[i] = od[i];
+    }
+}
+
+__global__ __launch_bounds__(TRI_NT) void k_eig_tri_lds(const EigMat *mats) {
+    __shared__ double red[TRI_NW];
+    tri_core<true>(mats[blockIdx.x], red);
+}
+
+__global__ __launch_bounds__(TRI_NT) void k_eig_tri_glb(const EigMat *mats) {
+    __shared__ double red[TRI_NW];
+    tri_core<false>(mats[blockIdx.x], red);
+}
+
+// ---------------------------------------------------------------------------
+// 2. divide and conquer on (dg, od)
+// Tree: nlev = floor(log2 n) levels, nl = 2^nlev leaves, leaf k = rows
+// [k n / nl, (k+1) n / nl) (1 or 2 rows).  The coupling b cut at each leaf
+// boundary mid is removed from the two adjacent diagonal entries (Cuppen),
+// T = diag(T1, T2) + |b| u u^T with u = e_{mid-1} + sign(b) e_mid.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline int dc_bound(int k, int n, int nlev) { return (int)(((long long)k * n) >> nlev); }
+__host__ __device__ inline int dc_levels(int n) {
+    int l = 0;
+    while ((2 << l) <= n) ++l;
+    return l;  // floor(log2 n)
+}
+
+__global__ __launch_bounds__(256) void k_dc_leaf(const EigMat *mats) {
+    const EigMat mt = mats[blockIdx.x];
+    const int n = mt.n;
+    double *ws = mt.ws;
+    const double *dg = vslot(ws, n, V_DG), *od = vslot(ws, n, V_OD);
+    double *D = vslot(ws, n, V_D), *beta = vslot(ws, n, V_BETA);
+    double *Vc = ws;
+    const int nlev = dc_levels(n), nl = 1 << nlev;
+    for (int k = threadIdx.x; k < nl; k += blockDim.x) {
+        const int a = dc_bound(k, n, nlev), b = dc_bound(k + 1, n, nlev);
+        double x = dg[a];
+        if (k > 0) {
+            x -= fabs(od[a - 1]);
+            beta[a] = od[a - 1];
+        }
+        if (b - a == 1) {
+            if (k < nl - 1) x -= fabs(od[a]);
+            D[a] = x;
+            Vc[(size_t)a * n + a] = 1.0;
+        } else {
+            double w = dg[a + 1];
+            if (k < nl - 1) w -= fabs(od[a + 1]);
+            const double y = od[a];
+            if (y == 0.0) {
+                D[a] = x;
+                D[a + 1] = w;
+                Vc[(size_t)a * n + a] = 1.0;
+                Vc[(size_t)a * n + a + 1] = 0.0;
+                Vc[(size_t)(a + 1) * n + a] = 0.0;
+                Vc[(size_t)(a + 1) * n + a + 1] = 1.0;
+            } else {
+                // Jacobi rotation of [[x, y], [y, w]]
+                const double th = (w - x) / (2.0 * y);
+                const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                D[a] = x - t * y;
+                D[a + 1] = w + t * y;
+                // eigenvector a = (c, -s), eigenvector a+1 = (s, c)
+                Vc[(size_t)a * n + a] = c;
+                Vc[(size_t)a * n + a + 1] = -s;
+                Vc[(size_t)(a + 1) * n + a] = s;
+                Vc[(size_t)(a + 1) * n + a + 1] = c;
+            }
+        }
+    }
+}
+
+// Sort, z, deflation and gather of one merge [a, b) with boundary mid.
+template <bool IN_LDS>
+__global__ __launch_bounds__(DEFL_NT) void k_dc_deflate(const DcMerge *merges, nrg_gemm_out *outs,
+                                                      nrg_gemm_term *terms) {
+    extern __shared__ double lds[];
+    __shared__ double red[DEFL_NT / 64];
+    __shared__ int s_info[2];
+    const DcMerge mg = merges[blockIdx.x];
+    const int n = mg.n, a = mg.a, h = mg.mid - mg.a, K = mg.b - mg.a;
+    double *ws = mg.ws;
+    double *Vc = ws, *S1 = ws + (size_t)n * n;
+    double *D = vslot(ws, n, V_D) + a;
+    const double beta = vslot(ws, n, V_BETA)[mg.mid];
+    const double rho = 2.0 * fabs(beta);
+    const double sg = beta >= 0.0 ? 1.0 : -1.0;
+    const double rs2 = 0.70710678118654752440;
+    const int tid = threadIdx.x;
+    double *draw, *ds, *zs;
+    int *perm, *pos;
+    if (IN_LDS) {
+        draw = lds;
+        ds = lds + K;
+        zs = lds + 2 * K;
+        perm = reinterpret_cast<int *>(lds + 3 * K);
+        pos = perm + K;
+    } else {
+        draw = vslot(ws, n, V_TS) + a;  // free until the secular stage
+        ds = vslot(ws, n, V_SDS) + a;
+        zs = vslot(ws, n, V_SZS) + a;
+        perm = islot(ws, n, V_PERM) + a;
+        pos = islot(ws, n, V_POS) + a;
+    }
+    for (int i = tid; i < K; i += DEFL_NT) draw[i] = D[i];
+    __syncthreads();
+    // stable ascending rank of the poles
+    for (int i = tid; i < K; i += DEFL_NT) {
+        const double di = draw[i];
+        int rk = 0;
+        for (int j = 0; j < K; ++j) {
+            const double dj = draw[j];
+            rk += (dj < di) || (dj == di && j < i);
+        }
+        perm[rk] = i;
+    }
+    __syncthreads();
+    double md = 0.0, mz = 0.0;
+    for (int q = tid; q < K; q += DEFL_NT) {
+        const int i = perm[q];
+        const size_t c = (size_t)(a + i);
+        const double z = (i < h) ? Vc[c * n + (mg.mid - 1)] * rs2 : sg * Vc[c * n + mg.mid] * rs2;
+        ds[q] = draw[i];
+        zs[q] = z;
+        md = fmax(md, fabs(draw[i]));
+        mz = fmax(mz, fabs(z));
+    }
+    md = block_max<DEFL_NT>(md, red);
+    mz = block_max<DEFL_NT>(mz, red);
+    const double tol = 8.0 * kEps * fmax(md, rho * mz);
+    int *rp = islot(ws, n, V_RP) + a, *rq = islot(ws, n, V_RQ) + a;
+    double *rc = vslot(ws, n, V_RC) + a, *rsn = vslot(ws, n, V_RS) + a;
+    if (tid == 0) {
+        // pos[q] = -1 marks a deflated pole
+        int pj = -1, nrot = 0;
+        for (int q = 0; q < K; ++q) {
+            pos[q] = 0;
+            if (rho * fabs(zs[q]) <= tol) {
+                pos[q] = -1;
+                continue;
+            }
+            if (pj < 0) {
+                pj = q;
+                continue;
+            }
+            double s = zs[pj], c = zs[q];
+            const double tau = hypot(c, s);
+            const double t = ds[q] - ds[pj];
+            c /= tau;
+            s = -s / tau;
+            if (fabs(t * c * s) <= tol) {
+                // close poles: rotate the pair so that pj's z component vanishes
+                zs[q] = tau;
+                zs[pj] = 0.0;
+                rp[nrot] = pj;
+                rq[nrot] = q;
+                rc[nrot] = c;
+                rsn[nrot] = s;
+                ++nrot;
+                const double tt = ds[pj] * c * c + ds[q] * s * s;
+                ds[q] = ds[pj] * s * s + ds[q] * c * c;
+                ds[pj] = tt;
+                pos[pj] = -1;
+            }
+            pj = q;
+        }
+        // positions: non-deflated first (ascending, as they stay sorted),
+        // then the deflated ones
+        int kp = 0;
+        for (int q = 0; q < K; ++q)
+            if (pos[q] >= 0) pos[q] = kp++;
+        int kd = kp;
+        for (int q = 0; q < K; ++q)
+            if (pos[q] < 0) pos[q] = kd++;
+        s_info[0] = kp;
+        s_info[1] = nrot;
+        int *meta = islot(ws, n, V_META) + a;
+        meta[0] = kp;
+        meta[1] = nrot;
+        vslot(ws, n, V_RHO)[a] = rho;
+        outs[mg.gemm].m = kp;
+        terms[mg.gemm].k = kp;
+    }
+    __syncthreads();
+    const int kp = s_info[0], nrot = s_info[1];
+    double *dd = vslot(ws, n, V_DD) + a, *zz = vslot(ws, n, V_ZZ) + a;
+    for (int q = tid; q < K; q += DEFL_NT) {
+        const int p = pos[q];
+        if (p < kp) {
+            dd[p] = ds[q];
+            zz[p] = zs[q];
+        } else {
+            D[p] = ds[q];  // deflated eigenvalue, final for this merge
+        }
+    }
+    // gather: row pos(q) of the block of S1 = eigenvector perm[q] restricted to
+    // its own half (zero in the other half)
+    double *G = S1 + (size_t)a * n + a;
+    for (size_t e = tid; e < (size_t)K * K; e += DEFL_NT) {
+        const int q = (int)(e / K), r = (int)(e - (size_t)q * K);
+        const int i = perm[q];
+        const bool own = (i < h) == (r < h);
+        G[(size_t)pos[q] * n + r] = own ? Vc[(size_t)(a + i) * n + a + r] : 0.0;
+    }
+    __syncthreads();
+    // deflation rotations, in order; each thread owns element r of every row
+    if (nrot > 0) {
+        for (int r = tid; r < K; r += DEFL_NT) {
+            for (int t = 0; t < nrot; ++t) {
+                double *x = G + (size_t)pos[rp[t]] * n + r;
+                double *y = G + (size_t)pos[rq[t]] * n + r;
+                const double c = rc[t], s = rsn[t];
+                const double xv = *x, yv = *y;
+                *x = c * xv + s * yv;
+                *y = c * yv - s * xv;
+            }
+        }
+    }
+    __syncthreads();
+    // deflated eigenvectors are final for this merge
+    for (size_t e = (size_t)kp * K + tid; e < (size_t)K * K; e += DEFL_NT) {
+        const int p = (int)(e / K), r = (int)(e - (size_t)p * K);
+        Vc[(size_t)(a + p) * n + a + r] = G[(size_t)p * n + r];
+    }
+}
+
+// Root j of 1/rho + sum_i zz_i^2 / (dd_i - lambda) = 0 (dd ascending,
+// rho > 0).  The root is found as tau = lambda - dd[org] where org is the
+// nearer pole (decided at the middle of the interval), by a bracketed
+// two-pole rational model: psi (poles <= j) and phi (poles > j) are each
+// replaced by c + s / (pole - x) matching value and slope at the current
+// point; the model's root (a quadratic) is taken if it lies inside the
+// bracket, otherwise the bracket is bisected.
+__device__ inline void secular_root(const double *dd, const double *zz, int K, double rho, int j, int &org_out,
+                                    double &tau_out) {
+    const double irho = 1.0 / rho;
+    if (K == 1) {
+        org_out = 0;
+        tau_out = rho * zz[0] * zz[0];
+        return;
+    }
+    const bool last = (j == K - 1);
+    double lo, hi;
+    int org;
+    if (!last) {
+        const double dj = dd[j];
+        const double mid = 0.5 * (dd[j + 1] - dj);
+        double f = irho;
+        for (int i = 0; i < K; ++i) f += zz[i] * zz[i] / ((dd[i] - dj) - mid);
+        if (f >= 0.0) {
+            org = j;
+            lo = 0.0;
+            hi = mid;
+        } else {
+            org = j + 1;
+            lo = -mid;
+            hi = 0.0;
+        }
+    } else {
+        double zt = 0.0;
+        for (int i = 0; i < K; ++i) zt += zz[i] * zz[i];
+        org = j;
+        lo = 0.0;
+        hi = rho * zt;
+    }
+    const double dorg = dd[org];
+    const double dk = dd[j] - dorg;
+    const double dk1 = last ? 0.0 : dd[j + 1] - dorg;
+    double tau = 0.5 * (lo + hi);
+    for (int it = 0; it < 64; ++it) {
+        double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
+        for (int i = 0; i <= j; ++i) {
+            const double r = 1.0 / ((dd[i] - dorg) - tau);
+            const double t = zz[i] * zz[i] * r;
+            psi += t;
+            dpsi += t * r;
+        }
+        for (int i = j + 1; i < K; ++i) {
+            const double r = 1.0 / ((dd[i] - dorg) - tau);
+            const double t = zz[i] * zz[i] * r;
+            phi += t;
+            dphi += t * r;
+        }
+        const double f = irho + psi + phi;
+        const double err = 8.0 * (fabs(psi) + fabs(phi) + irho) + fabs(tau) * (dpsi + dphi);
+        if (f < 0.0) lo = tau;
+        else hi = tau;
+        if (fabs(f) <= kEps * err || hi - lo <= 4.0 * kEps * fmax(fabs(lo), fabs(hi))) break;
+        const double ek = dk - tau;
+        const double bk = ek * ek * dpsi;
+        double C = irho + psi - ek * dpsi;
+        double nt;
+        if (!last) {
+            const double ek1 = dk1 - tau;
+            const double e1 = ek1 * ek1 * dphi;
+            C += phi - ek1 * dphi;
+            // C (dk - x)(dk1 - x) + bk (dk1 - x) + e1 (dk - x) = 0, one of dk, dk1 is 0
+            const double a2 = C;
+            const double a1 = -(C * (dk + dk1) + bk + e1);
+            const double a0 = bk * dk1 + e1 * dk;
+            nt = lo;  // invalid unless set below
+            if (a2 == 0.0) {
+                if (a1 != 0.0) nt = -a0 / a1;
+            } else {
+                const double disc = fmax(a1 * a1 - 4.0 * a2 * a0, 0.0);
+                const double q = -0.5 * (a1 + copysign(sqrt(disc), a1));
+                const double r1 = q / a2;
+                const double r2 = (q != 0.0) ? a0 / q : lo;
+                nt = (r2 > lo && r2 < hi) ? r2 : r1;
+            }
+        } else {
+            nt = (C > 0.0) ? bk / C : lo;
+        }
+        if (!(nt > lo && nt < hi)) nt = 0.5 * (lo + hi);
+        tau = nt;
+    }
+    org_out = org;
+    tau_out = tau;
+}
+
+__global__ __launch_bounds__(SEC_NT) void k_dc_secular(const DcMerge *merges, const DcChunk *chunks) {
+    __shared__ double s_tau[SEC_NT];
+    __shared__ double s_dorg[SEC_NT];
+    const DcChunk ck = chunks[blockIdx.x];
+    const DcMerge mg = merges[ck.merge];
+    const int n = mg.n, a = mg.a;
+    double *ws = mg.ws;
+    const int kp = islot(ws, n, V_META)[a];
+    if (ck.j0 >= kp) return;
+    const double rho = vslot(ws, n, V_RHO)[a];
+    const double *dd = vslot(ws, n, V_DD) + a, *zz = vslot(ws, n, V_ZZ) + a;
+    const int j = ck.j0 + threadIdx.x;
+    if (j < kp) {
+        int org;
+        double tau;
+        secular_root(dd, zz, kp, rho, j, org, tau);
+        vslot(ws, n, V_TS)[a + j] = tau;
+        islot(ws, n, V_ORG)[a + j] = org;
+        vslot(ws, n, V_D)[a + j] = dd[org] + tau;
+        s_tau[threadIdx.x] = tau;
+        s_dorg[threadIdx.x] = dd[org];
+    }
+    __syncthreads();
+    // Delta_ji = (dd_i - dd_org(j)) - tau_j, rows j of this chunk
+    double *S2 = ws + 2 * (size_t)n * n + (size_t)a * n + a;
+    const int jn = min(SEC_NT, kp - ck.j0);
+    for (int jj = 0; jj < jn; ++jj) {
+        const double tj = s_tau[jj], dj = s_dorg[jj];
+        double *row = S2 + (size_t)(ck.j0 + jj) * n;
+        for (int i = threadIdx.x; i < kp; i += SEC_NT) row[i] = (dd[i] - dj) - tj;
+    }
+}
+
+// z-hat_i^2 = -Delta_ii prod_{j != i} Delta_ji / (dd_i - dd_j) / rho
+__global__ __launch_bounds__(SEC_NT) void k_dc_zhat(const DcMerge *merges, const DcChunk *chunks) {
+    const DcChunk ck = chunks[blockIdx.x];
+    const DcMerge mg = merges[ck.merge];
+    const int n = mg.n, a = mg.a;
+    double *ws = mg.ws;
+    const int kp = islot(ws, n, V_META)[a];
+    const int i = ck.j0 + threadIdx.x;
+    if (i >= kp) return;
+    const double rho = vslot(ws, n, V_RHO)[a];
+    const double *dd = vslot(ws, n, V_DD) + a, *zz = vslot(ws, n, V_ZZ) + a;
+    const double *S2 = ws + 2 * (size_t)n * n + (size_t)a * n + a;
+    const double di = dd[i];
+    double w = S2[(size_t)i * n + i];
+    for (int j = 0; j < kp; ++j) {
+        if (j == i) continue;
+        w *= S2[(size_t)j * n + i] / (di - dd[j]);
+    }
+    vslot(ws, n, V_ZH)[a + i] = copysign(sqrt(fmax(-w / rho, 0.0)), zz[i]);
+}
+
+// Row j of S2 <- z-hat / Delta_j, normalised; one wavefront per row.
+__global__ __launch_bounds__(256) void k_dc_unorm(const DcMerge *merges, const DcChunk *chunks) {
+    const DcChunk ck = chunks[blockIdx.x];
+    const DcMerge mg = merges[ck.merge];
+    const int n = mg.n, a = mg.a;
+    double *ws = mg.ws;
+    const int kp = islot(ws, n, V_META)[a];
+    const int j = ck.j0 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (j >= kp) return;
+    const double *zh = vslot(ws, n, V_ZH) + a;
+    double *row = ws + 2 * (size_t)n * n + (size_t)(a + j) * n + a;
+    double s = 0.0;
+    for (int i = lane; i < kp; i += 64) {
+        const double u = zh[i] / row[i];
+        s += u * u;
+    }
+    s = wave_sum(s);
+    const double inv = 1.0 / sqrt(s);
+    for (int i = lane; i < kp; i += 64) row[i] = zh[i] / row[i] * inv;
+}
+
+// Stable ascending rank of the final eigenvalues; W[rank] = D.
+__global__ __launch_bounds__(256) void k_dc_rank(const EigMat *mats, const BtItem *items) {
+    const BtItem it = items[blockIdx.x];
+    const EigMat mt = mats[it.mat];
+    const int n = mt.n;
+    const int i = it.c0 + threadIdx.x;
+    if (i >= n) return;
+    const double *D = vslot(mt.ws, n, V_D);
+    const double di = D[i];
+    int rk = 0;
+    for (int j = 0; j < n; ++j) {
+        const double dj = D[j];
+        rk += (dj < di) || (dj == di && j < i);
+    }
+    islot(mt.ws, n, V_RANK)[i] = rk;
+    mt.W[rk] = di;
+}
+
+// ---------------------------------------------------------------------------
+// 3. back-transformation: x_c <- H_0 H_1 ... H_{n-3} x_c for every eigenvector
+//    x_c of the tridiagonal (row c of Vc); one wavefront per eigenvector, the
+//    vector in registers (NS x 64 elements), reflectors read from A.
+//    Output: row rank(c) of S1.
+// ---------------------------------------------------------------------------
+template <int NS>
+__global__ __launch_bounds__(256) void k_eig_bt(const EigMat *mats, const BtItem *items) {
+    const BtItem it = items[blockIdx.x];
+    const EigMat mt = mats[it.mat];
+    const int n = mt.n;
+    const int c = it.c0 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (c >= n) return;
+    double *ws = mt.ws;
+    const double *xc = ws + (size_t)c * n;
+    const double *tau = vslot(ws, n, V_TAU);
+    const double *A = mt.A;
+    double x[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int r = 64 * k + lane;
+        x[k] = (r < n) ? xc[r] : 0.0;
+    }
+    for (int j = n - 3; j >= 0; --j) {
+        const double t = tau[j];
+        if (t == 0.0) continue;
+        const double *v = A + (size_t)j * n;
+        double vr[NS];
+        double dot = 0.0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const int r = 64 * k + lane;
+            vr[k] = (r > j && r < n) ? v[r] : 0.0;
+            dot += vr[k] * x[k];
+        }
+        dot = wave_sum(dot) * t;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) x[k] -= dot * vr[k];
+    }
+    const int rk = islot(ws, n, V_RANK)[c];
+    double *out = ws + (size_t)n * n + (size_t)rk * n;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        const int r = 64 * k + lane;
+        if (r < n) out[r] = x[k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_eig_copy(const EigMat *mats, const BtItem *items) {
+    const BtItem it = items[blockIdx.x];
+    const EigMat mt = mats[it.mat];
+    const size_t nn = (size_t)mt.n * mt.n;
+    const double *src = mt.ws + nn;
+    const size_t e0 = (size_t)it.c0 * 256 * 16;
+    const size_t e1 = std::min(nn, e0 + 256 * 16);
+    for (size_t e = e0 + threadIdx.x; e < e1; e += 256) mt.A[e] = src[e];
+}
+
+constexpr int kBtSlots[] = {1, 2, 4, 8, 12, 16, 24, 32, 48, 64};
+constexpr int kBtVariants = sizeof(kBtSlots) / sizeof(kBtSlots[0]);
+
+template <int NS>
+static void launch_bt(hipStream_t s, int nitems, const EigMat *mats, const BtItem *items) {
+    hipLaunchKernelGGL(k_eig_bt<NS>, dim3(nitems), dim3(256), 0, s, mats, items);
+}
+static void launch_bt_variant(int v, hipStream_t s, int nitems, const EigMat *mats, const BtItem *items) {
+    switch (v) {
+        case 0: launch_bt<1>(s, nitems, mats, items); break;
+        case 1: launch_bt<2>(s, nitems, mats, items); break;
+        case 2: launch_bt<4>(s, nitems, mats, items); break;
+        case 3: launch_bt<8>(s, nitems, mats, items); break;
+        case 4: launch_bt<12>(s, nitems, mats, items); break;
+        case 5: launch_bt<16>(s, nitems, mats, items); break;
+        case 6: launch_bt<24>(s, nitems, mats, items); break;
+        case 7: launch_bt<32>(s, nitems, mats, items); break;
+        case 8: launch_bt<48>(s, nitems, mats, items); break;
+        default: launch_bt<64>(s, nitems, mats, items); break;
+    }
+}
+
+// descriptor arena assembled on the host and staged in one copy
+struct HostArena {
+    std::vector<char> buf;
+    template <class T>
+    size_t put(const std::vector<T> &v) {
+        size_t off = (buf.size() + 255) & ~(size_t)255;
+        buf.resize(off + sizeof(T) * v.size());
+        if (!v.empty()) memcpy(buf.data() + off, v.data(), sizeof(T) * v.size());
+        return off;
+    }
+};
+
+extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, int ntasks) {
+    if (ntasks <= 0) return NRG_OK;
+    // largest first: the LDS-sized tail of the tridiagonalisation is contiguous
+    std::vector<nrg_eig_task> tasks(tasks_in, tasks_in + ntasks);
+    std::stable_sort(tasks.begin(), tasks.end(), [](const nrg_eig_task &a, const nrg_eig_task &b) { return a.n > b.n; });
+    size_t tot = 0;
+    std::vector<size_t> off(ntasks);
+    for (int i = 0; i < ntasks; ++i) {
+        if (tasks[i].n <= 0 || !tasks[i].A || !tasks[i].W) return fail(c, NRG_ERR_ARG, "diag: bad task");
+        if (tasks[i].n > 64 * kBtSlots[kBtVariants - 1])
+            return fail(c, NRG_ERR_ARG, "diag: matrix dimension above " + std::to_string(64 * kBtSlots[kBtVariants - 1]));
+        off[i] = tot;
+        tot += eig_ws_doubles(tasks[i].n);
+    }
+    if (tot > c->ews_doubles) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->ews) HIPCHK(c, hipFree(c->ews));
+        c->ews = nullptr;
+        HIPCHK(c, hipMalloc(&c->ews, tot * sizeof(double)));
+        c->ews_doubles = tot;
+    }
+    // device pointers inside the staged arena are fixed up after staging, so
+    // descriptors hold workspace pointers directly
+    std::vector<EigMat> mats(ntasks);
+    for (int i = 0; i < ntasks; ++i) mats[i] = {tasks[i].A, tasks[i].W, c->ews + off[i], tasks[i].n, 0};
+    int nbig = 0;  // matrices with n >= 2 form the prefix
+    while (nbig < ntasks && tasks[nbig].n >= 2) ++nbig;
+
+    // divide-and-conquer tree, level by level over all matrices
+    int maxlev = 0;
+    for (int i = 0; i < nbig; ++i) maxlev = std::max(maxlev, dc_levels(tasks[i].n));
+    std::vector<DcMerge> merges;
+    std::vector<DcChunk> schunks, uchunks;
+    std::vector<nrg_gemm_out> gouts;
+    std::vector<nrg_gemm_term> gterms;
+    std::vector<GemmItem> gitems;
+    struct Level {
+        int m0, m1, s0, s1, u0, u1, g0, g1, maxK;
+        double flops;
+    };
+    std::vector<Level> levels(maxlev + 1);
+    for (int L = 1; L <= maxlev; ++L) {
+        Level lv{};
+        lv.m0 = (int)merges.size();
+        lv.s0 = (int)schunks.size();
+        lv.u0 = (int)uchunks.size();
+        lv.g0 = (int)gitems.size();
+        for (int i = 0; i < nbig; ++i) {
+            const int n = tasks[i].n, nlev = dc_levels(n);
+            if (L > nlev) continue;
+            double *ws = c->ews + off[i];
+            for (int k = 0; k < (1 << (nlev - L)); ++k) {
+                const int a = dc_bound(k << L, n, nlev), mid = dc_bound((k << L) + (1 << (L - 1)), n, nlev),
+                          b = dc_bound((k + 1) << L, n, nlev);
+                const int K = b - a, mi = (int)merges.size(), g = (int)gouts.size();
+                merges.push_back({ws, n, a, mid, b, g, 0});
+                nrg_gemm_out o{};
+                o.C = ws + (size_t)a * n + a;
+                o.m = K;
+                o.n = K;
+                o.ldc = n;
+                o.term_begin = g;
+                o.term_count = 1;
+                o.beta = 0.0;
+                gouts.push_back(o);
+                nrg_gemm_term t{};
+                t.A = ws + 2 * (size_t)n * n + (size_t)a * n + a;
+                t.B = ws + (size_t)n * n + (size_t)a * n + a;
+                t.s = nullptr;
+                t.alpha = 1.0;
+                t.k = K;
+                t.lda = n;
+                t.ldb = n;
+                gterms.push_back(t);
+                for (int j0 = 0; j0 < K; j0 += SEC_NT) schunks.push_back({mi, j0});
+                for (int j0 = 0; j0 < K; j0 += 4) uchunks.push_back({mi, j0});
+                for (int tm = 0; tm * GT < K; ++tm)
+                    for (int tn = 0; tn * GT < K; ++tn) gitems.push_back({g, tm, tn, 0});
+                lv.maxK = std::max(lv.maxK, K);
+                lv.flops += 2.0 * K * K * K;
+            }
+        }
+        lv.m1 = (int)merges.size();
+        lv.s1 = (int)schunks.size();
+        lv.u1 = (int)uchunks.size();
+        lv.g1 = (int)gitems.size();
+        levels[L] = lv;
+    }
+    // back-transformation items by register-slot variant; rank / copy items
+    std::vector<BtItem> bt[kBtVariants];
+    std::vector<BtItem> rk, cp;
+    for (int i = 0; i < nbig; ++i) {
+        const int n = tasks[i].n, ns = (n + 63) / 64;
+        int v = 0;
+        while (kBtSlots[v] < ns) ++v;
+        for (int c0 = 0; c0 < n; c0 += 4) bt[v].push_back({i, c0});
+        for (int c0 = 0; c0 < n; c0 += 256) rk.push_back({i, c0});
+        const size_t nn = (size_t)n * n;
+        for (size_t e = 0; e < nn; e += 256 * 16) cp.push_back({i, (int)(e / (256 * 16))});
+    }
+    HostArena ar;
+    const size_t o_mats = ar.put(mats), o_merges = ar.put(merges), o_sch = ar.put(schunks), o_uch = ar.put(uchunks),
+                 o_gouts = ar.put(gouts), o_gterms = ar.put(gterms), o_gitems = ar.put(gitems), o_rk = ar.put(rk),
+                 o_cp = ar.put(cp);
+    size_t o_bt[kBtVariants];
+    for (int v = 0; v < kBtVariants; ++v) o_bt[v] = ar.put(bt[v]);
+    void *d;
+    nrg_status s = stage(c, ar.buf.data(), ar.buf.size(), &d);
+    if (s) return s;
+    char *db = (char *)d;
+    const EigMat *d_mats = (const EigMat *)(db + o_mats);
+    const DcMerge *d_merges = (const DcMerge *)(db + o_merges);
+    const DcChunk *d_sch = (const DcChunk *)(db + o_sch), *d_uch = (const DcChunk *)(db + o_uch);
+    nrg_gemm_out *d_gouts = (nrg_gemm_out *)(db + o_gouts);
+    nrg_gemm_term *d_gterms = (nrg_gemm_term *)(db + o_gterms);
+    const GemmItem *d_gitems = (const GemmItem *)(db + o_gitems);
+
+    // 1. tridiagonalisation (global-vector variant for the large head)
+    {
+        double fl = 0, by = 0;
+        for (int i = 0; i < ntasks; ++i) {
+            const double n = tasks[i].n;
+            fl += 4.0 / 3.0 * n * n * n;
+            by += 16.0 / 3.0 * n * n * n;  // trailing matrix read + written per step
+        }
+        int split = ntasks;
+        while (split > 0 && tasks[split - 1].n <= TRI_LDS_N) --split;
+        int nl = 0;
+        for (int i = split; i < ntasks; ++i) nl = std::max(nl, tasks[i].n);
+        LaunchTimer lt(K_EIG_TRI, c->stream, fl, by);
+        if (split > 0) hipLaunchKernelGGL(k_eig_tri_glb, dim3(split), dim3(TRI_NT), 0, c->stream, d_mats);
+        if (split < ntasks)
+            hipLaunchKernelGGL(k_eig_tri_lds, dim3(ntasks - split), dim3(TRI_NT), 5 * (size_t)nl * sizeof(double),
+                               c->stream, d_mats + split);
+    }
+    HIPCHK(c, hipGetLastError());
+    if (nbig == 0) return NRG_OK;
+    // 2. divide and conquer
+    {
+        LaunchTimer lt(K_DC_LEAF, c->stream, 0.0, 0.0);
+        hipLaunchKernelGGL(k_dc_leaf, dim3(nbig), dim3(256), 0, c->stream, d_mats);
+    }
+    HIPCHK(c, hipGetLastError());
+    for (int L = 1; L <= maxlev; ++L) {
+        const Level &lv = levels[L];
+        const int nm = lv.m1 - lv.m0;
+        if (nm <= 0) continue;
+        {
+            LaunchTimer lt(K_DC_DEFLATE, c->stream, 0.0, 0.0);
+            if (lv.maxK <= DEFL_LDS_K)
+                hipLaunchKernelGGL(k_dc_deflate<true>, dim3(nm), dim3(DEFL_NT), 32 * (size_t)lv.maxK, c->stream,
+                                   d_merges + lv.m0, d_gouts, d_gterms);
+            else
+                hipLaunchKernelGGL(k_dc_deflate<false>, dim3(nm), dim3(DEFL_NT), 0, c->stream, d_merges + lv.m0,
+                                   d_gouts, d_gterms);
+        }
+        HIPCHK(c, hipGetLastError());
+        {
+            LaunchTimer lt(K_DC_SECULAR, c->stream, 0.0, 0.0);
+            hipLaunchKernelGGL(k_dc_secular, dim3(lv.s1 - lv.s0), dim3(SEC_NT), 0, c->stream, d_merges, d_sch + lv.s0);
+        }
+        {
+            LaunchTimer lt(K_DC_ZHAT, c->stream, 0.0, 0.0);
+            hipLaunchKernelGGL(k_dc_zhat, dim3(lv.s1 - lv.s0), dim3(SEC_NT), 0, c->stream, d_merges, d_sch + lv.s0);
+        }
+        {
+            LaunchTimer lt(K_DC_UNORM, c->stream, 0.0, 0.0);
+            hipLaunchKernelGGL(k_dc_unorm, dim3(lv.u1 - lv.u0), dim3(256), 0, c->stream, d_merges, d_uch + lv.u0);
+        }
+        HIPCHK(c, hipGetLastError());
+        s = gemm_launch_dev(c, K_GEMM_DC, d_gouts, d_gterms, d_gitems + lv.g0, lv.g1 - lv.g0, lv.flops, 0.0);
+        if (s) return s;
+    }
+    // 3. sort, back-transformation, copy into A
+    {
+        LaunchTimer lt(K_DC_RANK, c->stream, 0.0, 0.0);
+        hipLaunchKernelGGL(k_dc_rank, dim3(rk.size()), dim3(256), 0, c->stream, d_mats, (const BtItem *)(db + o_rk));
+    }
+    HIPCHK(c, hipGetLastError());
+    {
+        double fl = 0, by = 0;
+        for (int i = 0; i < nbig; ++i) {
+            const double n = tasks[i].n;
+            fl += 2.0 * n * n * n;
+            by += 8.0 * 2.0 * n * n;
+        }
+        LaunchTimer lt(K_EIG_BT, c->stream, fl, by);
+        for (int v = 0; v < kBtVariants; ++v)
+            if (!bt[v].empty())
+                launch_bt_variant(v, c->stream, (int)bt[v].size(), d_mats, (const BtItem *)(db + o_bt[v]));
+    }
+    HIPCHK(c, hipGetLastError());
+    {
+        double by = 0;
+        for (int i = 0; i < nbig; ++i) by += 16.0 * tasks[i].n * tasks[i].n;
+        LaunchTimer lt(K_EIG_COPY, c->stream, 0.0, by);
+        hipLaunchKernelGGL(k_eig_copy, dim3(cp.size()), dim3(256), 0, c->stream, d_mats, (const BtItem *)(db + o_cp));
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return NRG_OK;
+}

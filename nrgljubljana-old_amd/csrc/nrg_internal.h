@@ -1,0 +1,143 @@
+// nrg_internal.h - state and helpers shared by the translation units of
+// libnrg_gpu.so (context, error reporting, launch accounting, descriptor
+// staging, wave/block reductions, device-side grouped GEMM launches).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "nrg_gpu.h"
+
+struct nrg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // staging for descriptor arrays
+    void *desc = nullptr;
+    size_t desc_bytes = 0;
+    // eigensolver workspace
+    double *ews = nullptr;
+    size_t ews_doubles = 0;
+    int *dstatus = nullptr;
+};
+
+inline nrg_status fail(nrg_ctx *c, nrg_status code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HIPCHK(c, x)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess)                                                              \
+            return fail(c, NRG_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// per-kernel accounting (HIP events on the launch stream)
+// ---------------------------------------------------------------------------
+enum KernelId {
+    K_ZERO,
+    K_ASSEMBLE,
+    K_EIG_TRI,
+    K_DC_LEAF,
+    K_DC_DEFLATE,
+    K_DC_SECULAR,
+    K_DC_ZHAT,
+    K_DC_UNORM,
+    K_GEMM_DC,
+    K_DC_RANK,
+    K_EIG_BT,
+    K_EIG_COPY,
+    K_GEMM,
+    K_SPEC,
+    K_COUNT
+};
+extern const char *kKernelNames[K_COUNT];
+
+struct PendingLaunch {
+    int kid;
+    hipEvent_t a, b;
+    double flops, bytes;
+};
+extern bool g_timing;
+extern nrg_kernel_stat g_stats[K_COUNT];
+extern std::vector<PendingLaunch> g_pending;
+void timing_fold();
+
+struct LaunchTimer {
+    bool on;
+    PendingLaunch p;
+    hipStream_t s;
+    LaunchTimer(int kid, hipStream_t st, double flops, double bytes) : on(g_timing), s(st) {
+        if (!on) return;
+        p.kid = kid;
+        p.flops = flops;
+        p.bytes = bytes;
+        (void)hipEventCreate(&p.a);
+        (void)hipEventCreate(&p.b);
+        (void)hipEventRecord(p.a, s);
+    }
+    ~LaunchTimer() {
+        if (!on) return;
+        (void)hipEventRecord(p.b, s);
+        g_pending.push_back(p);
+    }
+};
+
+// Copies `bytes` of host descriptors into the context's staging buffer
+// (waits for the previous user of the buffer first).
+nrg_status stage(nrg_ctx *c, const void *host, size_t bytes, void **dev);
+
+// ---------------------------------------------------------------------------
+// reductions (wave64)
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double wave_max(double v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <int NT>
+__device__ inline double block_sum(double v, double *red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int k = 0; k < NT / 64; ++k) s += red[k];
+    __syncthreads();
+    return s;
+}
+template <int NT>
+__device__ inline double block_max(double v, double *red) {
+    v = wave_max(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    double s = red[0];
+    for (int k = 1; k < NT / 64; ++k) s = fmax(s, red[k]);
+    __syncthreads();
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// grouped GEMM work items (one 64x64 output tile each)
+// ---------------------------------------------------------------------------
+constexpr int GT = 64, GK = 16;
+struct GemmItem {
+    int out, tm, tn, pad;
+};
+// Launches the grouped GEMM on descriptors that already live in device
+// memory (outputs may have m / k patched by earlier kernels on the stream:
+// tiles past the patched m exit).
+nrg_status gemm_launch_dev(nrg_ctx *c, int kid, const nrg_gemm_out *d_outs, const nrg_gemm_term *d_terms,
+                           const GemmItem *d_items, int nitems, double flops, double bytes);

@@ -81,11 +81,33 @@ def test_diag_batch_degenerate_and_structured(ctx):
 
 
 def test_diag_batch_large(ctx):
+    # 1345: tridiagonalisation vectors in global memory; 2100: top merge of
+    # the divide and conquer above the LDS size of the deflation kernel
     rng = np.random.default_rng(3)
-    mats = [_sym(rng, 700), _sym(rng, 1100)]
+    mats = [_sym(rng, 700), _sym(rng, 1100), _sym(rng, 1345), _sym(rng, 2100)]
     out = ctx.diag_batch(mats)
     for A, (W, U) in zip(mats, out):
         _check_eig(A, W, U, tol=2e-13)
+
+
+def test_diag_batch_rejects_oversize(ctx):
+    with pytest.raises(RuntimeError):
+        ctx.diag_batch([np.zeros((4097, 4097))])
+
+
+def test_diag_batch_many_small_with_shared_spectrum(ctx):
+    # many blocks of one iteration: identical spectra in different blocks and
+    # clusters inside blocks (SU(2) multiplets of neighbouring subspaces)
+    rng = np.random.default_rng(6)
+    mats = []
+    for k in range(60):
+        n = int(rng.integers(1, 90))
+        lam = np.round(rng.uniform(0, 5, n), 1)
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        mats.append(Q @ np.diag(lam) @ Q.T)
+    out = ctx.diag_batch(mats)
+    for A, (W, U) in zip(mats, out):
+        _check_eig(A, W, U)
 
 
 def test_gemm_grouped_all_transposes_and_scales(ctx):
