@@ -65,6 +65,19 @@ nrg_status nrg_d2h_strided(nrg_ctx *ctx, double *dst, const double *src, size_t 
 /* Fills `count` doubles with zero on the stream (a kernel, not a memset node). */
 nrg_status nrg_zero(nrg_ctx *ctx, double *dst, size_t count);
 
+/* Batched transfers of many small strided device vectors (eigenvalues of all
+ * subspaces, density-matrix and operator diagonals): item k is `count`
+ * doubles at ptr[0], ptr[stride], ...; the host side is all items back to
+ * back.  One gather/scatter kernel and one copy per call.  No counterpart in
+ * the reference, which keeps these vectors in host memory. */
+typedef struct {
+    double *ptr;
+    int64_t count;
+    int64_t stride;
+} nrg_strided;
+nrg_status nrg_gather_d2h(nrg_ctx *ctx, const nrg_strided *items, int nitems, double *host);
+nrg_status nrg_scatter_h2d(nrg_ctx *ctx, const nrg_strided *items, int nitems, const double *host);
+
 /* ---- assembly ----------------------------------------------------------- */
 /* Replaces nrg_prepare_task_for_diag + makematrix (src/nrg.cc:2456-2478,
  * src/matrix.cc:86-192).  Each block is a dense symmetric d x d matrix that

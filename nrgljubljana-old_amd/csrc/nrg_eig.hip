@@ -510,11 +510,13 @@ __global__ __launch_bounds__(DEFL_NT) void k_dc_deflate(const DcMerge *merges, n
     // gather: row pos(q) of the block of S1 = eigenvector perm[q] restricted to
     // its own half (zero in the other half)
     double *G = S1 + (size_t)a * n + a;
-    for (size_t e = tid; e < (size_t)K * K; e += DEFL_NT) {
-        const int q = (int)(e / K), r = (int)(e - (size_t)q * K);
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int q = wave; q < K; q += DEFL_NT / 64) {
         const int i = perm[q];
-        const bool own = (i < h) == (r < h);
-        G[(size_t)pos[q] * n + r] = own ? Vc[(size_t)(a + i) * n + a + r] : 0.0;
+        const double *src = Vc + (size_t)(a + i) * n + a;
+        double *dst = G + (size_t)pos[q] * n;
+        const int r0 = (i < h) ? 0 : h, r1 = (i < h) ? h : K;
+        for (int r = lane; r < K; r += 64) dst[r] = (r >= r0 && r < r1) ? src[r] : 0.0;
     }
     __syncthreads();
     // deflation rotations, in order; each thread owns element r of every row
@@ -532,21 +534,23 @@ __global__ __launch_bounds__(DEFL_NT) void k_dc_deflate(const DcMerge *merges, n
     }
     __syncthreads();
     // deflated eigenvectors are final for this merge
-    for (size_t e = (size_t)kp * K + tid; e < (size_t)K * K; e += DEFL_NT) {
-        const int p = (int)(e / K), r = (int)(e - (size_t)p * K);
-        Vc[(size_t)(a + p) * n + a + r] = G[(size_t)p * n + r];
+    for (int p = kp + wave; p < K; p += DEFL_NT / 64) {
+        const double *src = G + (size_t)p * n;
+        double *dst = Vc + (size_t)(a + p) * n + a;
+        for (int r = lane; r < K; r += 64) dst[r] = src[r];
     }
 }
 
 // Root j of 1/rho + sum_i zz_i^2 / (dd_i - lambda) = 0 (dd ascending,
-// rho > 0).  The root is found as tau = lambda - dd[org] where org is the
-// nearer pole (decided at the middle of the interval), by a bracketed
-// two-pole rational model: psi (poles <= j) and phi (poles > j) are each
-// replaced by c + s / (pole - x) matching value and slope at the current
-// point; the model's root (a quadratic) is taken if it lies inside the
-// bracket, otherwise the bracket is bisected.
-__device__ inline void secular_root(const double *dd, const double *zz, int K, double rho, int j, int &org_out,
-                                    double &tau_out) {
+// rho > 0), computed by one wavefront (lanes share the sums over i).  The
+// root is found as tau = lambda - dd[org] where org is the nearer pole
+// (decided at the middle of the interval), by a bracketed two-pole rational
+// model: psi (poles <= j) and phi (poles > j) are each replaced by
+// c + s / (pole - x) matching value and slope at the current point; the
+// model's root (a quadratic) is taken if it lies inside the bracket,
+// otherwise the bracket is bisected.
+__device__ inline void secular_root(const double *dd, const double *zz, int K, double rho, int j, int lane,
+                                    int &org_out, double &tau_out) {
     const double irho = 1.0 / rho;
     if (K == 1) {
         org_out = 0;
@@ -559,8 +563,9 @@ __device__ inline void secular_root(const double *dd, const double *zz, int K, d
     if (!last) {
         const double dj = dd[j];
         const double mid = 0.5 * (dd[j + 1] - dj);
-        double f = irho;
-        for (int i = 0; i < K; ++i) f += zz[i] * zz[i] / ((dd[i] - dj) - mid);
+        double f = 0.0;
+        for (int i = lane; i < K; i += 64) f += zz[i] * zz[i] / ((dd[i] - dj) - mid);
+        f = irho + wave_sum(f);
         if (f >= 0.0) {
             org = j;
             lo = 0.0;
@@ -572,7 +577,8 @@ __device__ inline void secular_root(const double *dd, const double *zz, int K, d
         }
     } else {
         double zt = 0.0;
-        for (int i = 0; i < K; ++i) zt += zz[i] * zz[i];
+        for (int i = lane; i < K; i += 64) zt += zz[i] * zz[i];
+        zt = wave_sum(zt);
         org = j;
         lo = 0.0;
         hi = rho * zt;
@@ -583,18 +589,21 @@ __device__ inline void secular_root(const double *dd, const double *zz, int K, d
     double tau = 0.5 * (lo + hi);
     for (int it = 0; it < 64; ++it) {
         double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
-        for (int i = 0; i <= j; ++i) {
+        for (int i = lane; i < K; i += 64) {
             const double r = 1.0 / ((dd[i] - dorg) - tau);
             const double t = zz[i] * zz[i] * r;
-            psi += t;
-            dpsi += t * r;
+            if (i <= j) {
+                psi += t;
+                dpsi += t * r;
+            } else {
+                phi += t;
+                dphi += t * r;
+            }
         }
-        for (int i = j + 1; i < K; ++i) {
-            const double r = 1.0 / ((dd[i] - dorg) - tau);
-            const double t = zz[i] * zz[i] * r;
-            phi += t;
-            dphi += t * r;
-        }
+        psi = wave_sum(psi);
+        dpsi = wave_sum(dpsi);
+        phi = wave_sum(phi);
+        dphi = wave_sum(dphi);
         const double f = irho + psi + phi;
         const double err = 8.0 * (fabs(psi) + fabs(phi) + irho) + fabs(tau) * (dpsi + dphi);
         if (f < 0.0) lo = tau;
@@ -632,37 +641,29 @@ __device__ inline void secular_root(const double *dd, const double *zz, int K, d
     tau_out = tau;
 }
 
-__global__ __launch_bounds__(SEC_NT) void k_dc_secular(const DcMerge *merges, const DcChunk *chunks) {
-    __shared__ double s_tau[SEC_NT];
-    __shared__ double s_dorg[SEC_NT];
+// One wavefront per root: the root, then its row of Delta_ji = (dd_i -
+// dd_org(j)) - tau_j.
+__global__ __launch_bounds__(256) void k_dc_secular(const DcMerge *merges, const DcChunk *chunks) {
     const DcChunk ck = chunks[blockIdx.x];
     const DcMerge mg = merges[ck.merge];
     const int n = mg.n, a = mg.a;
     double *ws = mg.ws;
     const int kp = islot(ws, n, V_META)[a];
-    if (ck.j0 >= kp) return;
+    const int j = ck.j0 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (j >= kp) return;
     const double rho = vslot(ws, n, V_RHO)[a];
     const double *dd = vslot(ws, n, V_DD) + a, *zz = vslot(ws, n, V_ZZ) + a;
-    const int j = ck.j0 + threadIdx.x;
-    if (j < kp) {
-        int org;
-        double tau;
-        secular_root(dd, zz, kp, rho, j, org, tau);
+    int org;
+    double tau;
+    secular_root(dd, zz, kp, rho, j, lane, org, tau);
+    const double dorg = dd[org];
+    if (lane == 0) {
         vslot(ws, n, V_TS)[a + j] = tau;
         islot(ws, n, V_ORG)[a + j] = org;
-        vslot(ws, n, V_D)[a + j] = dd[org] + tau;
-        s_tau[threadIdx.x] = tau;
-        s_dorg[threadIdx.x] = dd[org];
+        vslot(ws, n, V_D)[a + j] = dorg + tau;
     }
-    __syncthreads();
-    // Delta_ji = (dd_i - dd_org(j)) - tau_j, rows j of this chunk
-    double *S2 = ws + 2 * (size_t)n * n + (size_t)a * n + a;
-    const int jn = min(SEC_NT, kp - ck.j0);
-    for (int jj = 0; jj < jn; ++jj) {
-        const double tj = s_tau[jj], dj = s_dorg[jj];
-        double *row = S2 + (size_t)(ck.j0 + jj) * n;
-        for (int i = threadIdx.x; i < kp; i += SEC_NT) row[i] = (dd[i] - dj) - tj;
-    }
+    double *row = ws + 2 * (size_t)n * n + (size_t)(a + j) * n + a;
+    for (int i = lane; i < kp; i += 64) row[i] = (dd[i] - dorg) - tau;
 }
 
 // z-hat_i^2 = -Delta_ii prod_{j != i} Delta_ji / (dd_i - dd_j) / rho
@@ -976,7 +977,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         HIPCHK(c, hipGetLastError());
         {
             LaunchTimer lt(K_DC_SECULAR, c->stream, 0.0, 0.0);
-            hipLaunchKernelGGL(k_dc_secular, dim3(lv.s1 - lv.s0), dim3(SEC_NT), 0, c->stream, d_merges, d_sch + lv.s0);
+            hipLaunchKernelGGL(k_dc_secular, dim3(lv.u1 - lv.u0), dim3(256), 0, c->stream, d_merges, d_uch + lv.u0);
         }
         {
             LaunchTimer lt(K_DC_ZHAT, c->stream, 0.0, 0.0);

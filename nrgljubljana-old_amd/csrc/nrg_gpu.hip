@@ -16,7 +16,7 @@
 
 const char *kKernelNames[K_COUNT] = {"k_zero",       "k_assemble", "k_eig_tri",  "k_dc_leaf", "k_dc_deflate",
                                      "k_dc_secular", "k_dc_zhat",  "k_dc_unorm", "k_gemm_dc", "k_dc_rank",
-                                     "k_eig_bt",     "k_eig_copy", "k_gemm",     "k_spec_bin"};
+                                     "k_eig_bt",     "k_eig_copy", "k_gemm",     "k_spec_bin", "k_gather_scatter"};
 bool g_timing = false;
 nrg_kernel_stat g_stats[K_COUNT];
 std::vector<PendingLaunch> g_pending;
@@ -59,6 +59,77 @@ __global__ void k_zero(double *p, size_t n) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     size_t st = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += st) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// batched strided transfers (through one contiguous staging buffer)
+// ---------------------------------------------------------------------------
+struct XItem {
+    double *ptr;
+    long long count, stride, off;
+};
+struct XChunk {
+    int item, pad;
+    long long e0, e1;
+};
+constexpr long long kXChunk = 4096;
+
+__global__ __launch_bounds__(256) void k_gather(const XItem *items, const XChunk *chunks, double *buf) {
+    const XChunk ck = chunks[blockIdx.x];
+    const XItem it = items[ck.item];
+    for (long long e = ck.e0 + threadIdx.x; e < ck.e1; e += 256) buf[it.off + e] = it.ptr[e * it.stride];
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const XItem *items, const XChunk *chunks, const double *buf) {
+    const XChunk ck = chunks[blockIdx.x];
+    const XItem it = items[ck.item];
+    for (long long e = ck.e0 + threadIdx.x; e < ck.e1; e += 256) it.ptr[e * it.stride] = buf[it.off + e];
+}
+
+// Stages the item list and sizes the device / pinned buffers; returns the
+// total element count.
+static nrg_status xfer_prepare(nrg_ctx *c, const nrg_strided *items, int nitems, size_t *total, XItem **d_items,
+                               XChunk **d_chunks, int *nchunks) {
+    std::vector<XItem> xi(nitems);
+    std::vector<XChunk> xc;
+    long long off = 0;
+    for (int k = 0; k < nitems; ++k) {
+        if (items[k].count < 0 || (items[k].count > 0 && (!items[k].ptr || items[k].stride < 1)))
+            return fail(c, NRG_ERR_ARG, "transfer: bad item");
+        xi[k] = {items[k].ptr, items[k].count, items[k].stride, off};
+        for (long long e = 0; e < items[k].count; e += kXChunk)
+            xc.push_back({k, 0, e, std::min<long long>(items[k].count, e + kXChunk)});
+        off += items[k].count;
+    }
+    *total = (size_t)off;
+    *nchunks = (int)xc.size();
+    if (!off) return NRG_OK;
+    if ((size_t)off > c->xbuf_doubles) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->xbuf) HIPCHK(c, hipFree(c->xbuf));
+        c->xbuf = nullptr;
+        const size_t nd = std::max<size_t>((size_t)off * 2, 1 << 16);
+        HIPCHK(c, hipMalloc(&c->xbuf, nd * sizeof(double)));
+        c->xbuf_doubles = nd;
+    }
+    if ((size_t)off > c->hpin_doubles) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->hpin) HIPCHK(c, hipHostFree(c->hpin));
+        c->hpin = nullptr;
+        const size_t nd = std::max<size_t>((size_t)off * 2, 1 << 16);
+        HIPCHK(c, hipHostMalloc((void **)&c->hpin, nd * sizeof(double), hipHostMallocDefault));
+        c->hpin_doubles = nd;
+    }
+    const size_t ib = sizeof(XItem) * xi.size();
+    std::vector<char> buf(ib + sizeof(XChunk) * xc.size());
+    memcpy(buf.data(), xi.data(), ib);
+    memcpy(buf.data() + ib, xc.data(), sizeof(XChunk) * xc.size());
+    void *d;
+    nrg_status s = stage(c, buf.data(), buf.size(), &d);
+    if (s) return s;
+    *d_items = (XItem *)d;
+    *d_chunks = (XChunk *)((char *)d + ib);
+    return NRG_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -284,6 +355,8 @@ void nrg_ctx_destroy(nrg_ctx *c) {
     if (c->desc) (void)hipFree(c->desc);
     if (c->ews) (void)hipFree(c->ews);
     if (c->dstatus) (void)hipFree(c->dstatus);
+    if (c->xbuf) (void)hipFree(c->xbuf);
+    if (c->hpin) (void)hipHostFree(c->hpin);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -349,6 +422,45 @@ nrg_status nrg_d2h_strided(nrg_ctx *c, double *dst, const double *src, size_t co
     if (!count) return NRG_OK;
     HIPCHK(c, hipMemcpy2DAsync(dst, sizeof(double), src, stride * sizeof(double), sizeof(double), count,
                                hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return NRG_OK;
+}
+
+nrg_status nrg_gather_d2h(nrg_ctx *c, const nrg_strided *items, int nitems, double *host) {
+    if (nitems <= 0) return NRG_OK;
+    size_t tot = 0;
+    XItem *di = nullptr;
+    XChunk *dc = nullptr;
+    int nch = 0;
+    nrg_status s = xfer_prepare(c, items, nitems, &tot, &di, &dc, &nch);
+    if (s || !tot) return s;
+    {
+        LaunchTimer lt(K_XFER, c->stream, 0.0, 16.0 * tot);
+        hipLaunchKernelGGL(k_gather, dim3(nch), dim3(256), 0, c->stream, di, dc, c->xbuf);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->hpin, c->xbuf, tot * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    memcpy(host, c->hpin, tot * sizeof(double));
+    return NRG_OK;
+}
+
+nrg_status nrg_scatter_h2d(nrg_ctx *c, const nrg_strided *items, int nitems, const double *host) {
+    if (nitems <= 0) return NRG_OK;
+    size_t tot = 0;
+    XItem *di = nullptr;
+    XChunk *dc = nullptr;
+    int nch = 0;
+    nrg_status s = xfer_prepare(c, items, nitems, &tot, &di, &dc, &nch);
+    if (s || !tot) return s;
+    memcpy(c->hpin, host, tot * sizeof(double));
+    HIPCHK(c, hipMemcpyAsync(c->xbuf, c->hpin, tot * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    {
+        LaunchTimer lt(K_XFER, c->stream, 0.0, 16.0 * tot);
+        hipLaunchKernelGGL(k_scatter, dim3(nch), dim3(256), 0, c->stream, di, dc, c->xbuf);
+    }
+    HIPCHK(c, hipGetLastError());
+    // the pinned buffer and the staged items are reused by the next call
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return NRG_OK;
 }

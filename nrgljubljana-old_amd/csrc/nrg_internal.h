@@ -24,6 +24,11 @@ struct nrg_ctx {
     double *ews = nullptr;
     size_t ews_doubles = 0;
     int *dstatus = nullptr;
+    // batched transfers: device staging and pinned host staging
+    double *xbuf = nullptr;
+    size_t xbuf_doubles = 0;
+    double *hpin = nullptr;
+    size_t hpin_doubles = 0;
 };
 
 inline nrg_status fail(nrg_ctx *c, nrg_status code, const std::string &msg) {
@@ -56,6 +61,7 @@ enum KernelId {
     K_EIG_COPY,
     K_GEMM,
     K_SPEC,
+    K_XFER,
     K_COUNT
 };
 extern const char *kKernelNames[K_COUNT];

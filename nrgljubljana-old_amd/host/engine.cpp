@@ -26,16 +26,47 @@ namespace nrg {
 // device helpers
 // ---------------------------------------------------------------------------
 Gpu::Gpu(int device) { check(nrg_ctx_create(&ctx, device), "nrg_ctx_create"); }
-Gpu::~Gpu() { nrg_ctx_destroy(ctx); }
+Gpu::~Gpu() {
+    if (ctx) (void)nrg_sync(ctx);
+    for (auto &kv : pool_size) (void)nrg_dev_free(ctx, kv.first);
+    nrg_ctx_destroy(ctx);
+}
 void Gpu::check(nrg_status s, const char *what) const {
     if (s != NRG_OK) die(std::string(what) + " failed: " + (ctx ? nrg_last_error(ctx) : "no GPU context"));
 }
+static size_t pool_class(size_t n) {
+    size_t c = 1 << 12;
+    while (c < n) c <<= 1;
+    if (c <= (1 << 12)) return c;
+    // 8 classes between c/2 and c
+    const size_t step = c / 16;
+    return (n + step - 1) / step * step;
+}
 double *Gpu::alloc(size_t n) {
+    const size_t sz = pool_class(std::max<size_t>(n, 1));
+    auto it = pool_free.find(sz);
+    if (it != pool_free.end()) {
+        double *p = it->second;
+        pool_free.erase(it);
+        return p;
+    }
     void *p = nullptr;
-    check(nrg_dev_alloc(ctx, n * sizeof(double), &p), "nrg_dev_alloc");
+    check(nrg_dev_alloc(ctx, sz * sizeof(double), &p), "nrg_dev_alloc");
+    pool_size[(double *)p] = sz;
     return (double *)p;
 }
-void Gpu::free(void *p) { check(nrg_dev_free(ctx, p), "nrg_dev_free"); }
+void Gpu::free(void *p) {
+    if (!p) return;
+    auto it = pool_size.find((double *)p);
+    if (it == pool_size.end()) die("Gpu::free: pointer not from the pool");
+    pool_free.insert({it->second, it->first});
+}
+void Gpu::gather(const std::vector<nrg_strided> &items, double *host) {
+    check(nrg_gather_d2h(ctx, items.data(), (int)items.size(), host), "nrg_gather_d2h");
+}
+void Gpu::scatter(const std::vector<nrg_strided> &items, const double *host) {
+    check(nrg_scatter_h2d(ctx, items.data(), (int)items.size(), host), "nrg_scatter_h2d");
+}
 void Gpu::h2d(double *d, const double *h, size_t n) { check(nrg_h2d(ctx, d, h, n * sizeof(double)), "nrg_h2d"); }
 void Gpu::d2h(double *h, const double *d, size_t n) { check(nrg_d2h(ctx, h, d, n * sizeof(double)), "nrg_d2h"); }
 
@@ -460,18 +491,30 @@ struct Engine {
         auto fo = [&](auto x) { fcustom << std::setw(P.width_custom) << std::setprecision(P.prec_custom) << x << ' '; };
         fcustom << ' ';
         fo(Tnow);
+        std::vector<nrg_strided> items;
+        size_t tot = 0;
         for (auto &name : singlet_names) {
             const DevOp &op = ops.at(name);
-            double tr = 0;
             for (auto &kv : vals) {
                 const int n = (int)kv.second.size();
                 if (!n) continue;
                 auto it = op.find({kv.first, kv.first});
                 if (it == op.end()) die("calc_trace_singlet: missing block");
-                std::vector<double> dg(n);
-                gpu.check(nrg_d2h_strided(gpu.ctx, dg.data(), it->second.p, n, it->second.ld + 1), "d2h diag");
+                items.push_back({it->second.p, n, it->second.ld + 1});
+                tot += n;
+            }
+        }
+        std::vector<double> dg(tot);
+        gpu.gather(items, dg.data());
+        size_t pos = 0;
+        for (size_t k = 0; k < singlet_names.size(); ++k) {
+            double tr = 0;
+            for (auto &kv : vals) {
+                const int n = (int)kv.second.size();
+                if (!n) continue;
                 double s = 0;
-                for (int r = 0; r < n; ++r) s += std::exp(-P.betabar * kv.second[r]) * dg[r];
+                for (int r = 0; r < n; ++r) s += std::exp(-P.betabar * kv.second[r]) * dg[pos + r];
+                pos += n;
                 tr += T.mult(kv.first) * s;
             }
             fo(tr / Z);
@@ -709,9 +752,18 @@ struct Engine {
                 }
                 std::stable_sort(tasks.begin(), tasks.end(), [](const nrg_eig_task &a, const nrg_eig_task &b) { return a.n > b.n; });
                 gpu.check(nrg_diag_batch(gpu.ctx, tasks.data(), (int)tasks.size()), "nrg_diag_batch");
+                std::vector<nrg_strided> items;
+                size_t tot = 0;
                 for (auto &kv : it.subs) {
-                    kv.second.value.resize(kv.second.d);
-                    gpu.d2h(kv.second.value.data(), kv.second.W, kv.second.d);
+                    items.push_back({kv.second.W, kv.second.d, 1});
+                    tot += kv.second.d;
+                }
+                std::vector<double> all(tot);
+                gpu.gather(items, all.data());
+                size_t pos = 0;
+                for (auto &kv : it.subs) {
+                    kv.second.value.assign(all.begin() + pos, all.begin() + pos + kv.second.d);
+                    pos += kv.second.d;
                 }
             });
             // ---- ground state, truncation (src/nrg.cc:2881-2929)
@@ -724,13 +776,19 @@ struct Engine {
             truncate_prepare(it, last, false);
             totalenergy += Egs * it.scale;
             std::map<Invar, std::vector<double>> allv, keptv;
-            for (auto &kv : it.subs) {
-                Sub &s = kv.second;
-                s.absenergy.resize(s.d);
-                for (int r = 0; r < s.d; ++r) s.absenergy[r] = totalenergy + s.value[r] * it.scale;
-                gpu.h2d(s.W, s.value.data(), s.d);  // shifted values become E_prev
-                allv[kv.first] = s.value;
-                keptv[kv.first] = std::vector<double>(s.value.begin(), s.value.begin() + s.kept);
+            {
+                std::vector<nrg_strided> items;
+                std::vector<double> all;
+                for (auto &kv : it.subs) {
+                    Sub &s = kv.second;
+                    s.absenergy.resize(s.d);
+                    for (int r = 0; r < s.d; ++r) s.absenergy[r] = totalenergy + s.value[r] * it.scale;
+                    items.push_back({s.W, s.d, 1});  // shifted values become E_prev
+                    all.insert(all.end(), s.value.begin(), s.value.end());
+                    allv[kv.first] = s.value;
+                    keptv[kv.first] = std::vector<double>(s.value.begin(), s.value.begin() + s.kept);
+                }
+                gpu.scatter(items, all.data());
             }
             if (fenergies.is_open()) {
                 fenergies << "\n===== Iteration number: " << N << "\n";
@@ -773,14 +831,22 @@ struct Engine {
 
     // ------------------------------------------------------------ DM sweeps
     void rho_trace_check(const RhoMap &r, double expected, const char *what, int N) {
-        double tr = 0;
+        std::vector<nrg_strided> items;
+        size_t tot = 0;
         for (auto &kv : r) {
             if (!kv.second.dim) continue;
-            std::vector<double> dg(kv.second.dim);
-            if (kv.second.diag) gpu.d2h(dg.data(), kv.second.p, kv.second.dim);
-            else gpu.check(nrg_d2h_strided(gpu.ctx, dg.data(), kv.second.p, kv.second.dim, kv.second.dim + 1), "d2h");
+            items.push_back({kv.second.p, kv.second.dim, kv.second.diag ? 1 : kv.second.dim + 1});
+            tot += kv.second.dim;
+        }
+        std::vector<double> dg(tot);
+        gpu.gather(items, dg.data());
+        double tr = 0;
+        size_t pos = 0;
+        for (auto &kv : r) {
+            if (!kv.second.dim) continue;
             double s = 0;
-            for (double x : dg) s += x;
+            for (int i = 0; i < kv.second.dim; ++i) s += dg[pos + i];
+            pos += kv.second.dim;
             tr += T.mult(kv.first) * s;
         }
         const double rel = (tr - expected) / expected;
@@ -844,16 +910,19 @@ struct Engine {
         double ZZ = 0;
         for (auto &kv : it.subs)
             for (int r = 0; r < kv.second.kept; ++r) ZZ += T.mult(kv.first) * std::exp(-kv.second.value[r] * scale_last / P.T);
+        std::vector<nrg_strided> items;
+        std::vector<double> h;
         for (auto &kv : it.subs) {
             const int k = kv.second.kept;
             Rho r;
             r.dim = k;
             r.p = rho_arena->alloc(std::max<size_t>(1, (size_t)k * k));
-            std::vector<double> h((size_t)k * k, 0.0);
-            for (int i = 0; i < k; ++i) h[(size_t)i * k + i] = std::exp(-kv.second.value[i] * scale_last / P.T) / ZZ;
-            if (k) gpu.h2d(r.p, h.data(), h.size());
+            if (k) gpu.check(nrg_zero(gpu.ctx, r.p, (size_t)k * k), "nrg_zero");
+            for (int i = 0; i < k; ++i) h.push_back(std::exp(-kv.second.value[i] * scale_last / P.T) / ZZ);
+            if (k) items.push_back({r.p, k, k + 1});
             rho_init[kv.first] = r;
         }
+        gpu.scatter(items, h.data());
         rho_trace_check(rho_init, 1.0, "init_rho", Nmax - 1);
     }
 
@@ -893,22 +962,28 @@ struct Engine {
     RhoMap init_rho_FDM(int N) {
         RhoMap r;
         const bool last = (N + 1 == Nmax);
+        std::vector<nrg_strided> items;
+        std::vector<double> h;
         for (auto &kv : iters[N].subs) {
             const int mn = last ? 0 : kv.second.kept, mx = kv.second.d;
-            std::vector<double> h(mx, 0.0);
-            for (int i = mn; i < mx; ++i) {
-                const long double Eabs = (long double)kv.second.absenergy[i] - (long double)GSenergy;
-                long double v = expl(-Eabs / (long double)P.T) * ((long double)wn[N] / (long double)ZnD[N]);
-                if (!std::isfinite((double)v)) v = 0;
-                h[i] = (double)v;
+            for (int i = 0; i < mx; ++i) {
+                double hv = 0.0;
+                if (i >= mn) {
+                    const long double Eabs = (long double)kv.second.absenergy[i] - (long double)GSenergy;
+                    long double v = expl(-Eabs / (long double)P.T) * ((long double)wn[N] / (long double)ZnD[N]);
+                    if (!std::isfinite((double)v)) v = 0;
+                    hv = (double)v;
+                }
+                h.push_back(hv);
             }
             Rho x;
             x.dim = mx;
             x.diag = true;
             x.p = rho_arena->alloc(mx);
-            gpu.h2d(x.p, h.data(), mx);
+            if (mx) items.push_back({x.p, mx, 1});
             r[kv.first] = x;
         }
+        gpu.scatter(items, h.data());
         return r;
     }
 
@@ -1096,12 +1171,16 @@ struct Engine {
             truncate_prepare(it, last, true);
             Arena eabs(&gpu);
             eabs.chunk = 1 << 16;
-            for (auto &kv : it.subs) {
-                Sub &s = kv.second;
-                std::vector<double> e(s.d);
-                for (int r = 0; r < s.d; ++r) e[r] = s.absenergy[r] - GSenergy;
-                s.Eabs = eabs.alloc(s.d);
-                gpu.h2d(s.Eabs, e.data(), s.d);
+            {
+                std::vector<nrg_strided> items;
+                std::vector<double> e;
+                for (auto &kv : it.subs) {
+                    Sub &s = kv.second;
+                    for (int r = 0; r < s.d; ++r) e.push_back(s.absenergy[r] - GSenergy);
+                    s.Eabs = eabs.alloc(s.d);
+                    items.push_back({s.Eabs, s.d, 1});
+                }
+                gpu.scatter(items, e.data());
             }
             st.old_op_arena = std::move(st.op_arena);
             st.op_arena.reset(new Arena(&gpu));
@@ -1229,8 +1308,8 @@ struct Engine {
             f << "\"" << kv.first << "\": {\"s\": " << kv.second.t << ", \"calls\": " << kv.second.n << "}";
         }
         f << "}, \"kernels\": {";
-        nrg_kernel_stat ks[16];
-        const int nk = std::min(nrg_timing_get(ks, 16), 16);
+        nrg_kernel_stat ks[64];
+        const int nk = std::min(nrg_timing_get(ks, 64), 64);
         for (int k = 0; k < nk; ++k) {
             if (k) f << ", ";
             f << "\"" << ks[k].name << "\": {\"launches\": " << ks[k].launches << ", \"ms\": " << ks[k].ms

@@ -79,8 +79,15 @@ struct Tables {
 };
 
 // ---- device helpers -------------------------------------------------------
+// Device memory comes from a pool of size classes (8 per octave): a freed
+// block is kept for the next request of its class instead of being returned
+// with hipFree, which would synchronise the device.  All work is on one
+// stream, so a block reused by a later launch is ordered after its previous
+// users.
 struct Gpu {
     nrg_ctx *ctx = nullptr;
+    std::multimap<size_t, double *> pool_free;
+    std::map<double *, size_t> pool_size;
     explicit Gpu(int device);
     ~Gpu();
     void check(nrg_status s, const char *what) const;
@@ -88,6 +95,9 @@ struct Gpu {
     void free(void *p);
     void h2d(double *d, const double *h, size_t n);
     void d2h(double *h, const double *d, size_t n);
+    // one transfer for many (strided) device vectors, host side back to back
+    void gather(const std::vector<nrg_strided> &items, double *host);
+    void scatter(const std::vector<nrg_strided> &items, const double *host);
 };
 
 // Bump allocator over large device chunks, released together.

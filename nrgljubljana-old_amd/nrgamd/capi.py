@@ -13,7 +13,7 @@ PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(PKG, "lib", "libnrg_gpu.so")
 
 EXPORTS = ["nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_sync", "nrg_dev_alloc", "nrg_dev_free",
-           "nrg_h2d", "nrg_d2h", "nrg_d2h_strided", "nrg_zero", "nrg_assemble_batch", "nrg_diag_batch",
+           "nrg_h2d", "nrg_d2h", "nrg_d2h_strided", "nrg_zero", "nrg_gather_d2h", "nrg_scatter_h2d", "nrg_assemble_batch", "nrg_diag_batch",
            "nrg_gemm_grouped", "nrg_spec_bin", "nrg_timing_enable", "nrg_timing_reset", "nrg_timing_get"]
 
 
@@ -23,10 +23,10 @@ class KernelStat(C.Structure):
 
 
 def kernel_stats(lib):
-    arr = (KernelStat * 16)()
-    n = lib.nrg_timing_get(arr, 16)
+    arr = (KernelStat * 64)()
+    n = lib.nrg_timing_get(arr, 64)
     return {arr[i].name.decode(): dict(launches=arr[i].launches, ms=arr[i].ms, flops=arr[i].flops,
-                                       bytes=arr[i].bytes) for i in range(min(n, 16))}
+                                       bytes=arr[i].bytes) for i in range(min(n, 64))}
 
 
 class AsmBlock(C.Structure):

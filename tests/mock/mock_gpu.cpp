@@ -48,6 +48,16 @@ nrg_status nrg_d2h_strided(nrg_ctx *, double *d, const double *s, size_t n, size
     for (size_t i = 0; i < n; ++i) d[i] = s[i * st];
     return NRG_OK;
 }
+nrg_status nrg_gather_d2h(nrg_ctx *, const nrg_strided *it, int n, double *host) {
+    for (int k = 0; k < n; ++k)
+        for (int64_t e = 0; e < it[k].count; ++e) *host++ = it[k].ptr[e * it[k].stride];
+    return NRG_OK;
+}
+nrg_status nrg_scatter_h2d(nrg_ctx *, const nrg_strided *it, int n, const double *host) {
+    for (int k = 0; k < n; ++k)
+        for (int64_t e = 0; e < it[k].count; ++e) it[k].ptr[e * it[k].stride] = *host++;
+    return NRG_OK;
+}
 nrg_status nrg_zero(nrg_ctx *, double *d, size_t n) {
     std::fill(d, d + n, 0.0);
     return NRG_OK;
