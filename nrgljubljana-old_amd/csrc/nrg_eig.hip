@@ -79,7 +79,7 @@ struct EigMat {
     double *W;
     double *ws;
     int n;
-    int pad;
+    int pad;  // 1: reduced by a team (reflectors in S2)
 };
 
 struct DcMerge {
@@ -313,14 +313,218 @@ __device__ __forceinline__ void tri_core(const EigMat &mt, double *red) {
     }
 }
 
-__global__ __launch_bounds__(TRI_NT) void k_eig_tri_lds(const EigMat *mats) {
-    __shared__ double red[TRI_NW];
-    tri_core<true>(mats[blockIdx.x], red);
+__global__ void k_fill_int(int *p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
-__global__ __launch_bounds__(TRI_NT) void k_eig_tri_glb(const EigMat *mats) {
+// ---------------------------------------------------------------------------
+// 1b. team tridiagonalisation: T workgroups share one matrix; column slice
+//     s = [1 + s W, 1 + (s+1) W) of A belongs to rank s mod T for the whole
+//     reduction (cyclic, so the active columns stay spread over the team).
+//     Per step j (j = -1 .. n-3) every workgroup holds v_j (redundant
+//     copies), reads p_j = tau_j A22 v_j and row j+1 of A (as left by step
+//     j-1), computes w = p - K v, the updated row j+1 and from it v_{j+1}
+//     (all redundant, O(n)), and finally updates its own columns of rows /
+//     columns j+2.. while accumulating p_{j+1} for them.  One barrier per
+//     step.
+//     Only p and the next row cross workgroups; they are written and read
+//     with device-scope atomic stores / loads, which are coherent across
+//     XCDs without cache write-back.  Everything else a workgroup touches is
+//     its own (its columns of A, private vectors), so the barrier needs no
+//     fence: each wave waits for its stores, then the workgroups meet on a
+//     relaxed atomic counter.  Reflectors go to S2 (rank 0) and are copied
+//     into the rows of A by the next kernel.
+// ---------------------------------------------------------------------------
+constexpr int TEAM_W = 16;                   // columns per slice
+constexpr int TEAM_RL = TRI_NT / TEAM_W;     // row lanes
+constexpr int TEAM_MIN_N = 128;              // smaller matrices: one workgroup
+constexpr int TEAM_MAX = 32;                 // workgroups of one XCD
+constexpr int N_XCD = 8;                     // workgroup b runs on XCD b mod 8
+constexpr long long TEAM_SPIN_LIMIT = 1LL << 22;  // polls (~1 us each): a few seconds
+
+struct TriJob {
+    int mat, T;
+};
+
+__device__ inline void st_dev(double *p, double v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline double ld_dev(const double *p) {
+    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Team barrier with a bounded wait: returns false (and flags the status) if
+// the team did not arrive, so that every workgroup still terminates.
+__device__ inline bool team_barrier(int *cnt, int target, int *status, int *s_flag) {
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's device-scope stores have completed
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(cnt, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        long long spins = 0;
+        int ok = 1;
+        while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > TEAM_SPIN_LIMIT || __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ok = 0;
+                atomicCAS(status, 0, NRG_ERR_HIP);
+                break;
+            }
+        }
+        *s_flag = ok;
+    }
+    __syncthreads();
+    return *s_flag != 0;
+}
+
+template <bool IN_LDS>
+__device__ __forceinline__ void tri_team(const EigMat &mt, int rank, int T, int *bar, int *status, double *red,
+                                         double *red2, int *s_flag) {
+    extern __shared__ double lds[];
+    const int n = mt.n;
+    double *__restrict__ A = mt.A;
+    double *ws = mt.ws;
+    const int tid = threadIdx.x;
+    double *vecs = IN_LDS ? lds : ws + (size_t)n * n + (size_t)rank * 4 * n;  // private copies in S1
+    double *va = vecs, *vb = vecs + n, *wv = vecs + 2 * n, *xr = vecs + 3 * n;
+    double *pv[2] = {vslot(ws, n, V_PV), vslot(ws, n, V_DGT)};
+    double *rb[2] = {vslot(ws, n, V_ODT), vslot(ws, n, V_VB)};
+    double *Y = ws + 2 * (size_t)n * n;  // reflectors (row j, columns j+1..)
+    double *tau = vslot(ws, n, V_TAU);
+    double *gdg = vslot(ws, n, V_DG), *god = vslot(ws, n, V_OD);
+    const int tcol = tid % TEAM_W, rl = tid / TEAM_W;
+    const int nslices = (n - 1 + TEAM_W - 1) / TEAM_W;
+    double tj = 0.0;  // tau_j
+    int phase = 0;
+    for (int j = -1; j <= n - 3; ++j) {
+        const int m = n - j - 1;  // length of v_j / p_j; row j+1 has m entries from column j+1
+        if (rank == 0 && j >= 0 && tj != 0.0)
+            for (int k = tid; k < m; k += TRI_NT) Y[(size_t)j * n + j + 1 + k] = va[k];
+        // row j+1 as left by step j-1 (row 0: the assembled matrix)
+        if (j < 0) {
+            for (int k = tid; k < m; k += TRI_NT) xr[k] = A[k];
+        } else {
+            const double *rg = rb[(j + 2) & 1];
+            for (int k = tid; k < m; k += TRI_NT) xr[k] = ld_dev(rg + k);
+        }
+        if (tj != 0.0) {
+            const double *pg = pv[(j + 2) & 1];
+            double pp = 0.0;
+            for (int k = tid; k < m; k += TRI_NT) {
+                const double x = ld_dev(pg + k);
+                wv[k] = x;
+                pp += va[k] * x;
+            }
+            const double Kc = 0.5 * tj * tri_block_sum(pp, red);
+            for (int k = tid; k < m; k += TRI_NT) wv[k] -= Kc * va[k];
+            __syncthreads();
+            // x = row j+1 after step j's update
+            const double v0 = va[0], w0 = wv[0];
+            for (int k = tid; k < m; k += TRI_NT) xr[k] -= v0 * wv[k] + w0 * va[k];
+        }
+        __syncthreads();
+        double psum = 0.0;
+        for (int k = 2 + tid; k < m; k += TRI_NT) psum += xr[k] * xr[k];
+        const double sigma = tri_block_sum(psum, red);
+        const double x0 = xr[0], x1 = (m >= 2) ? xr[1] : 0.0;
+        const bool has_next = (j + 1) <= n - 3;
+        double tn = 0.0, alpha = x1;
+        if (has_next && sigma != 0.0) {
+            alpha = -copysign(sqrt(x1 * x1 + sigma), x1);
+            const double v0 = x1 - alpha;
+            tn = 2.0 / (v0 * v0 + sigma);
+            for (int k = tid; k < m - 1; k += TRI_NT) vb[k] = (k == 0) ? v0 : xr[k + 1];
+        }
+        if (rank == 0 && tid == 0) {
+            gdg[j + 1] = x0;
+            god[j + 1] = alpha;
+            if (has_next) tau[j + 1] = tn;
+        }
+        __syncthreads();
+        // own columns: update rows/cols j+2.., publish row j+2, accumulate p_{j+1}
+        double *rnext = rb[(j + 3) & 1];
+        double *pnext = pv[(j + 3) & 1];
+        const int s0 = (j + 1) / TEAM_W;  // first slice with a column >= j+2
+        for (int sl = s0 + ((rank - s0 % T) + T) % T; sl < nslices; sl += T) {
+            const int c = 1 + sl * TEAM_W + tcol;
+            double acc = 0.0;
+            if (c >= j + 2 && c < n) {
+                const int cc = c - j - 1;  // index of column c in v_j / w_j
+                const double wc = (tj != 0.0) ? wv[cc] : 0.0, vc = (tj != 0.0) ? va[cc] : 0.0;
+                for (int r = j + 2 + rl; r < n; r += TEAM_RL) {
+                    double *q = A + (size_t)r * n + c;
+                    double a = *q;
+                    if (tj != 0.0) {
+                        a -= va[r - j - 1] * wc + wv[r - j - 1] * vc;
+                        *q = a;
+                    }
+                    if (r == j + 2) st_dev(rnext + (c - j - 2), a);
+                    acc += a * vb[r - j - 2];
+                }
+            }
+            if (tn != 0.0) {
+                red2[rl * TEAM_W + tcol] = acc;
+                __syncthreads();
+                if (tid < TEAM_W) {
+                    const int cw = 1 + sl * TEAM_W + tid;
+                    if (cw >= j + 2 && cw < n) {
+                        double sacc = 0.0;
+                        for (int q = 0; q < TEAM_RL; ++q) sacc += red2[q * TEAM_W + tid];
+                        st_dev(pnext + (cw - j - 2), tn * sacc);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        ++phase;
+        if (!team_barrier(bar, phase * T, status, s_flag)) return;
+        // v_{j+1} becomes current
+        if (has_next) {
+            for (int k = tid; k < m - 1; k += TRI_NT) va[k] = vb[k];
+            __syncthreads();
+        }
+        tj = tn;
+    }
+    if (rank == 0 && tid == 0) {
+        gdg[n - 1] = ld_dev(rb[n & 1]);  // A[n-1][n-1], published by the last pass
+        god[n - 1] = 0.0;
+    }
+}
+
+// One persistent launch for all matrices of a batch: workgroups of XCD x
+// (blockIdx mod 8) draw tickets from queue x; a job of T tickets is a team
+// of T workgroups (T = 1: the one-workgroup kernel).
+__global__ __launch_bounds__(TRI_NT) void k_eig_tri_coop(const EigMat *mats, const TriJob *jobs, const int *tbase,
+                                                         const int *qjob, int *counters, int *status) {
     __shared__ double red[TRI_NW];
-    tri_core<false>(mats[blockIdx.x], red);
+    __shared__ double red2[TRI_NT];
+    __shared__ int s_ticket, s_flag;
+    const int x = blockIdx.x % N_XCD;
+    int *ticket = counters + x;           // counters[0..7]: next ticket per queue
+    int *bars = counters + N_XCD;         // counters[8 + job]: team barrier
+    const int j0 = qjob[x], j1 = qjob[x + 1];
+    const int t0 = tbase[j0], total = tbase[j1] - t0;
+    for (;;) {
+        if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1);
+        __syncthreads();
+        const int t = s_ticket;
+        __syncthreads();
+        if (t >= total) break;
+        int lo = j0, hi = j1;  // last job of the queue with tbase <= t0 + t
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (tbase[mid] <= t0 + t) lo = mid;
+            else hi = mid;
+        }
+        const TriJob jb = jobs[lo];
+        const EigMat mt = mats[jb.mat];
+        const int rank = t0 + t - tbase[lo];
+        if (jb.T == 1) {
+            if (mt.n <= TRI_LDS_N) tri_core<true>(mt, red);
+            else tri_core<false>(mt, red);
+        } else {
+            if (mt.n <= TRI_LDS_N) tri_team<true>(mt, rank, jb.T, bars + lo, status, red, red2, &s_flag);
+            else tri_team<false>(mt, rank, jb.T, bars + lo, status, red, red2, &s_flag);
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -344,6 +548,12 @@ __global__ __launch_bounds__(256) void k_dc_leaf(const EigMat *mats) {
     const double *dg = vslot(ws, n, V_DG), *od = vslot(ws, n, V_OD);
     double *D = vslot(ws, n, V_D), *beta = vslot(ws, n, V_BETA);
     double *Vc = ws;
+    if (mt.pad) {
+        // reflectors of a team reduction: S2 row j, columns j+1.. -> row j of A
+        const double *Y = ws + 2 * (size_t)n * n;
+        for (int j = 0; j < n - 2; ++j)
+            for (int k = j + 1 + threadIdx.x; k < n; k += blockDim.x) mt.A[(size_t)j * n + k] = Y[(size_t)j * n + k];
+    }
     const int nlev = dc_levels(n), nl = 1 << nlev;
     for (int k = threadIdx.x; k < nl; k += blockDim.x) {
         const int a = dc_bound(k, n, nlev), b = dc_bound(k + 1, n, nlev);
@@ -918,7 +1128,61 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         const size_t nn = (size_t)n * n;
         for (size_t e = 0; e < nn; e += 256 * 16) cp.push_back({i, (int)(e / (256 * 16))});
     }
+    // tridiagonalisation jobs: ticket ranges, team sizes
+    if (c->tri_capacity <= 0) {
+        int per_cu = 0, dev = 0, ncu = 0;
+        HIPCHK(c, hipGetDevice(&dev));
+        HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_coop, TRI_NT,
+                                                               5 * (size_t)TRI_LDS_N * sizeof(double)));
+        c->tri_capacity = std::max(1, per_cu * ncu);
+    }
+    // Team sizes, then longest-processing-time assignment of the jobs to the
+    // XCD queues (work = estimated time x workgroups; a team stays on one XCD)
+    const int tri_grid = c->tri_capacity;
+    const int per_xcd = std::max(1, tri_grid / N_XCD);
+    std::vector<int> teamT(ntasks, 1);
+    std::vector<double> cost(ntasks);
+    for (int i = 0; i < ntasks; ++i) {
+        const double n = tasks[i].n;
+        int T = 1;
+        if (tasks[i].n >= TEAM_MIN_N) T = std::min({(tasks[i].n - 1 + TEAM_W - 1) / TEAM_W, TEAM_MAX, per_xcd});
+        if (T < 2) T = 1;
+        teamT[i] = T;
+        // microseconds: one workgroup streams the trailing matrix per step;
+        // a team pays a barrier per step plus its share of the matrix
+        cost[i] = (T == 1) ? 0.047 * n * n : n * (3.0 + 1.1e-4 * n * n / T);
+        if (T > 1) mats[i].pad = 1;  // reflectors to copy from S2
+    }
+    std::vector<int> order(ntasks);
+    for (int i = 0; i < ntasks; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<std::vector<int>> queue(N_XCD);
+    std::vector<double> load(N_XCD, 0.0);
+    for (int i : order) {
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        queue[x].push_back(i);
+        load[x] += cost[i] * teamT[i] / per_xcd;
+    }
+    std::vector<TriJob> jobs;
+    std::vector<int> tbase(1, 0), qjob(1, 0);
+    for (int x = 0; x < N_XCD; ++x) {
+        for (int i : queue[x]) {
+            jobs.push_back({i, teamT[i]});
+            tbase.push_back(tbase.back() + teamT[i]);
+        }
+        qjob.push_back((int)jobs.size());
+    }
+    const size_t ncnt = N_XCD + (size_t)ntasks;  // ticket counters + one barrier per job
+    if (ncnt > c->cnt_ints) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->cnt) HIPCHK(c, hipFree(c->cnt));
+        c->cnt = nullptr;
+        c->cnt_ints = std::max<size_t>(ncnt * 2, 1024);
+        HIPCHK(c, hipMalloc(&c->cnt, c->cnt_ints * sizeof(int)));
+    }
     HostArena ar;
+    const size_t o_jobs = ar.put(jobs), o_tb = ar.put(tbase), o_qj = ar.put(qjob);
     const size_t o_mats = ar.put(mats), o_merges = ar.put(merges), o_sch = ar.put(schunks), o_uch = ar.put(uchunks),
                  o_gouts = ar.put(gouts), o_gterms = ar.put(gterms), o_gitems = ar.put(gitems), o_rk = ar.put(rk),
                  o_cp = ar.put(cp);
@@ -929,13 +1193,17 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     if (s) return s;
     char *db = (char *)d;
     const EigMat *d_mats = (const EigMat *)(db + o_mats);
+    const TriJob *d_jobs = (const TriJob *)(db + o_jobs);
+    const int *d_tb = (const int *)(db + o_tb);
+    const int *d_qj = (const int *)(db + o_qj);
     const DcMerge *d_merges = (const DcMerge *)(db + o_merges);
     const DcChunk *d_sch = (const DcChunk *)(db + o_sch), *d_uch = (const DcChunk *)(db + o_uch);
     nrg_gemm_out *d_gouts = (nrg_gemm_out *)(db + o_gouts);
     nrg_gemm_term *d_gterms = (nrg_gemm_term *)(db + o_gterms);
     const GemmItem *d_gitems = (const GemmItem *)(db + o_gitems);
 
-    // 1. tridiagonalisation (global-vector variant for the large head)
+    // 1. tridiagonalisation: one persistent launch; matrices of
+    //    TEAM_MIN_N and more get a team of workgroups (one per TEAM_W columns)
     {
         double fl = 0, by = 0;
         for (int i = 0; i < ntasks; ++i) {
@@ -943,15 +1211,15 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             fl += 4.0 / 3.0 * n * n * n;
             by += 16.0 / 3.0 * n * n * n;  // trailing matrix read + written per step
         }
-        int split = ntasks;
-        while (split > 0 && tasks[split - 1].n <= TRI_LDS_N) --split;
-        int nl = 0;
-        for (int i = split; i < ntasks; ++i) nl = std::max(nl, tasks[i].n);
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, c->stream, c->cnt, (int)ncnt);
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->dstatus, 1);
+        HIPCHK(c, hipGetLastError());
         LaunchTimer lt(K_EIG_TRI, c->stream, fl, by);
-        if (split > 0) hipLaunchKernelGGL(k_eig_tri_glb, dim3(split), dim3(TRI_NT), 0, c->stream, d_mats);
-        if (split < ntasks)
-            hipLaunchKernelGGL(k_eig_tri_lds, dim3(ntasks - split), dim3(TRI_NT), 5 * (size_t)nl * sizeof(double),
-                               c->stream, d_mats + split);
+        // grid <= co-resident capacity (one workgroup per CU at this size) and
+        // nothing else runs on the stream, so every team assembles; the
+        // barrier's bounded wait reports an error otherwise
+        hipLaunchKernelGGL(k_eig_tri_coop, dim3(tri_grid), dim3(TRI_NT), 5 * (size_t)TRI_LDS_N * sizeof(double),
+                           c->stream, d_mats, d_jobs, d_tb, d_qj, c->cnt, c->dstatus);
     }
     HIPCHK(c, hipGetLastError());
     if (nbig == 0) return NRG_OK;
@@ -1017,6 +1285,9 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         hipLaunchKernelGGL(k_eig_copy, dim3(cp.size()), dim3(256), 0, c->stream, d_mats, (const BtItem *)(db + o_cp));
     }
     HIPCHK(c, hipGetLastError());
+    int st = 0;
+    HIPCHK(c, hipMemcpyAsync(&st, c->dstatus, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (st) return fail(c, NRG_ERR_HIP, "diag: a tridiagonalisation team did not assemble in time");
     return NRG_OK;
 }

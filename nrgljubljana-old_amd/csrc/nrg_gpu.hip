@@ -354,6 +354,7 @@ void nrg_ctx_destroy(nrg_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->desc) (void)hipFree(c->desc);
     if (c->ews) (void)hipFree(c->ews);
+    if (c->cnt) (void)hipFree(c->cnt);
     if (c->dstatus) (void)hipFree(c->dstatus);
     if (c->xbuf) (void)hipFree(c->xbuf);
     if (c->hpin) (void)hipHostFree(c->hpin);

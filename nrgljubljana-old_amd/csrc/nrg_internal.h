@@ -24,6 +24,10 @@ struct nrg_ctx {
     double *ews = nullptr;
     size_t ews_doubles = 0;
     int *dstatus = nullptr;
+    // tridiagonalisation: ticket / barrier counters, co-resident workgroups
+    int *cnt = nullptr;
+    size_t cnt_ints = 0;
+    int tri_capacity = 0;
     // batched transfers: device staging and pinned host staging
     double *xbuf = nullptr;
     size_t xbuf_doubles = 0;
