@@ -114,8 +114,8 @@ nrg_status nrg_assemble_batch(nrg_ctx *ctx, const nrg_asm_block *blocks, int nbl
  * dimension n) holds the full symmetric matrix; on exit row r of A is the
  * normalised eigenvector for W[r], with W ascending (the layout of
  * Eigen::matrix0 / Eigen::value, src/nrg.cc:285-376).  Workspace is internal
- * (3 n^2 + O(n) doubles per matrix).  n is limited to 4096; larger matrices
- * return NRG_ERR_ARG.  NRG_ERR_NOCONV is reserved (the divide-and-conquer
+ * (3 n^2 + O(n) doubles per matrix).  n is limited to 32768; larger matrices
+ * return NRG_ERR_ARG before any device access.  NRG_ERR_NOCONV is reserved (the divide-and-conquer
  * solver has no failing iteration). */
 typedef struct {
     double *A;

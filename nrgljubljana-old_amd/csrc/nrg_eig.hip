@@ -318,27 +318,33 @@ __global__ void k_fill_int(int *p, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// 1b. team tridiagonalisation: T workgroups share one matrix; column slice
-//     s = [1 + s W, 1 + (s+1) W) of A belongs to rank s mod T for the whole
-//     reduction (cyclic, so the active columns stay spread over the team).
+// 1b. team tridiagonalisation: T workgroups share one matrix; row r of A
+//     belongs to rank r mod T for the whole reduction (cyclic, so the active
+//     rows stay spread over the team).  A is stored in full, so the
+//     matrix-vector product is a dot product per owned row.
 //     Per step j (j = -1 .. n-3) every workgroup holds v_j (redundant
 //     copies), reads p_j = tau_j A22 v_j and row j+1 of A (as left by step
 //     j-1), computes w = p - K v, the updated row j+1 and from it v_{j+1}
-//     (all redundant, O(n)), and finally updates its own columns of rows /
-//     columns j+2.. while accumulating p_{j+1} for them.  One barrier per
-//     step.
+//     (all redundant, O(n)), and finally updates its own rows (columns j+2..)
+//     while computing p_{j+1} for them.  One barrier per step.
 //     Only p and the next row cross workgroups; they are written and read
 //     with device-scope atomic stores / loads, which are coherent across
 //     XCDs without cache write-back.  Everything else a workgroup touches is
-//     its own (its columns of A, private vectors), so the barrier needs no
+//     its own (its rows of A, private vectors), so the barrier needs no
 //     fence: each wave waits for its stores, then the workgroups meet on a
 //     relaxed atomic counter.  Reflectors go to S2 (rank 0) and are copied
 //     into the rows of A by the next kernel.
 // ---------------------------------------------------------------------------
-constexpr int TEAM_W = 16;                   // columns per slice
-constexpr int TEAM_RL = TRI_NT / TEAM_W;     // row lanes
+constexpr int TEAM_W = 16;                   // rows per workgroup when sizing a team
+constexpr int TEAM_RW = 2;                   // rows a wavefront updates together
+constexpr int TEAM_CU = 4;                   // column chunks of 64 per round
+constexpr int TEAM_LDS_N = 6800;             // v, v', w of the team in LDS (160 KB)
+constexpr size_t TRI_COOP_LDS = 3 * (size_t)TEAM_LDS_N * sizeof(double);  // >= 5 x TRI_LDS_N doubles
+static_assert(TRI_COOP_LDS >= 5 * (size_t)TRI_LDS_N * sizeof(double), "one-workgroup vectors must fit");
 constexpr int TEAM_MIN_N = 128;              // smaller matrices: one workgroup
 constexpr int TEAM_MAX = 32;                 // workgroups of one XCD
+constexpr int TEAM_GLOBAL_N = 1024;          // larger matrices: teams across XCDs
+constexpr int TEAM_GLOBAL_MAX = 128;
 constexpr int N_XCD = 8;                     // workgroup b runs on XCD b mod 8
 constexpr long long TEAM_SPIN_LIMIT = 1LL << 22;  // polls (~1 us each): a few seconds
 
@@ -376,21 +382,20 @@ __device__ inline bool team_barrier(int *cnt, int target, int *status, int *s_fl
 
 template <bool IN_LDS>
 __device__ __forceinline__ void tri_team(const EigMat &mt, int rank, int T, int *bar, int *status, double *red,
-                                         double *red2, int *s_flag) {
+                                         int *s_flag) {
     extern __shared__ double lds[];
     const int n = mt.n;
     double *__restrict__ A = mt.A;
     double *ws = mt.ws;
     const int tid = threadIdx.x;
-    double *vecs = IN_LDS ? lds : ws + (size_t)n * n + (size_t)rank * 4 * n;  // private copies in S1
-    double *va = vecs, *vb = vecs + n, *wv = vecs + 2 * n, *xr = vecs + 3 * n;
-    double *pv[2] = {vslot(ws, n, V_PV), vslot(ws, n, V_DGT)};
-    double *rb[2] = {vslot(ws, n, V_ODT), vslot(ws, n, V_VB)};
+    double *priv = ws + (size_t)n * n + (size_t)rank * 4 * n;  // private area in S1
+    double *vecs = IN_LDS ? lds : priv;
+    double *va = vecs, *vb = vecs + n, *wv = vecs + 2 * n, *xr = priv + 3 * n;
+    double *const pv0 = vslot(ws, n, V_PV), *const pv1 = vslot(ws, n, V_DGT);  // p, by step parity
+    double *const rb0 = vslot(ws, n, V_ODT), *const rb1 = vslot(ws, n, V_VB);  // next row, by parity
     double *Y = ws + 2 * (size_t)n * n;  // reflectors (row j, columns j+1..)
     double *tau = vslot(ws, n, V_TAU);
     double *gdg = vslot(ws, n, V_DG), *god = vslot(ws, n, V_OD);
-    const int tcol = tid % TEAM_W, rl = tid / TEAM_W;
-    const int nslices = (n - 1 + TEAM_W - 1) / TEAM_W;
     double tj = 0.0;  // tau_j
     int phase = 0;
     for (int j = -1; j <= n - 3; ++j) {
@@ -401,11 +406,11 @@ __device__ __forceinline__ void tri_team(const EigMat &mt, int rank, int T, int 
         if (j < 0) {
             for (int k = tid; k < m; k += TRI_NT) xr[k] = A[k];
         } else {
-            const double *rg = rb[(j + 2) & 1];
+            const double *rg = ((j + 2) & 1) ? rb1 : rb0;
             for (int k = tid; k < m; k += TRI_NT) xr[k] = ld_dev(rg + k);
         }
         if (tj != 0.0) {
-            const double *pg = pv[(j + 2) & 1];
+            const double *pg = ((j + 2) & 1) ? pv1 : pv0;
             double pp = 0.0;
             for (int k = tid; k < m; k += TRI_NT) {
                 const double x = ld_dev(pg + k);
@@ -438,39 +443,85 @@ __device__ __forceinline__ void tri_team(const EigMat &mt, int rank, int T, int 
             if (has_next) tau[j + 1] = tn;
         }
         __syncthreads();
-        // own columns: update rows/cols j+2.., publish row j+2, accumulate p_{j+1}
-        double *rnext = rb[(j + 3) & 1];
-        double *pnext = pv[(j + 3) & 1];
-        const int s0 = (j + 1) / TEAM_W;  // first slice with a column >= j+2
-        for (int sl = s0 + ((rank - s0 % T) + T) % T; sl < nslices; sl += T) {
-            const int c = 1 + sl * TEAM_W + tcol;
-            double acc = 0.0;
-            if (c >= j + 2 && c < n) {
-                const int cc = c - j - 1;  // index of column c in v_j / w_j
-                const double wc = (tj != 0.0) ? wv[cc] : 0.0, vc = (tj != 0.0) ? va[cc] : 0.0;
-                for (int r = j + 2 + rl; r < n; r += TEAM_RL) {
-                    double *q = A + (size_t)r * n + c;
-                    double a = *q;
-                    if (tj != 0.0) {
-                        a -= va[r - j - 1] * wc + wv[r - j - 1] * vc;
-                        *q = a;
-                    }
-                    if (r == j + 2) st_dev(rnext + (c - j - 2), a);
-                    acc += a * vb[r - j - 2];
+        double *rnext = ((j + 3) & 1) ? rb1 : rb0;
+        double *pnext = ((j + 3) & 1) ? pv1 : pv0;
+        // own rows (cyclic, r mod T = rank): update columns j+2..,
+        // p_{j+1}[r] = tn * (row r . v_{j+1}) (A is symmetric), and the owner
+        // of row j+2 publishes it for the next step.  A wavefront takes up to
+        // TEAM_RW of its rows at once, so each column's vector entries are
+        // read once for all of them and their loads are in flight together.
+        {
+            const int first = j + 2;
+            const int wave = tid >> 6, lane = tid & 63;
+            const int r0 = first + ((rank - first % T) + T) % T;
+            const int rstep = TRI_NW * T;
+            for (int rg = r0 + wave * T; rg < n; rg += TEAM_RW * rstep) {
+                double vr[TEAM_RW], wr[TEAM_RW], acc[TEAM_RW];
+#pragma unroll
+                for (int k = 0; k < TEAM_RW; ++k) {
+                    const int r = rg + k * rstep;
+                    const bool ok = r < n && tj != 0.0;
+                    vr[k] = ok ? va[r - j - 1] : 0.0;
+                    wr[k] = ok ? wv[r - j - 1] : 0.0;
+                    acc[k] = 0.0;
                 }
-            }
-            if (tn != 0.0) {
-                red2[rl * TEAM_W + tcol] = acc;
-                __syncthreads();
-                if (tid < TEAM_W) {
-                    const int cw = 1 + sl * TEAM_W + tid;
-                    if (cw >= j + 2 && cw < n) {
-                        double sacc = 0.0;
-                        for (int q = 0; q < TEAM_RL; ++q) sacc += red2[q * TEAM_W + tid];
-                        st_dev(pnext + (cw - j - 2), tn * sacc);
+                int cidx = first + lane;
+                // TEAM_CU column chunks x TEAM_RW rows of independent loads per round
+                for (; cidx + 64 * (TEAM_CU - 1) < n; cidx += 64 * TEAM_CU) {
+                    double a[TEAM_CU][TEAM_RW];
+#pragma unroll
+                    for (int u = 0; u < TEAM_CU; ++u)
+#pragma unroll
+                        for (int k = 0; k < TEAM_RW; ++k) {
+                            const int r = rg + k * rstep;
+                            a[u][k] = (r < n) ? A[(size_t)r * n + cidx + 64 * u] : 0.0;
+                        }
+#pragma unroll
+                    for (int u = 0; u < TEAM_CU; ++u) {
+                        const int cu = cidx + 64 * u;
+                        const double vc = (tj != 0.0) ? va[cu - j - 1] : 0.0;
+                        const double wc = (tj != 0.0) ? wv[cu - j - 1] : 0.0;
+                        const double bc = (tn != 0.0) ? vb[cu - first] : 0.0;
+#pragma unroll
+                        for (int k = 0; k < TEAM_RW; ++k) {
+                            const int r = rg + k * rstep;
+                            if (r < n) {
+                                if (tj != 0.0) {
+                                    a[u][k] -= vr[k] * wc + wr[k] * vc;
+                                    A[(size_t)r * n + cu] = a[u][k];
+                                }
+                                if (r == first) st_dev(rnext + (cu - first), a[u][k]);
+                                acc[k] += a[u][k] * bc;
+                            }
+                        }
                     }
                 }
-                __syncthreads();
+                for (; cidx < n; cidx += 64) {
+                    const double vc = (tj != 0.0) ? va[cidx - j - 1] : 0.0;
+                    const double wc = (tj != 0.0) ? wv[cidx - j - 1] : 0.0;
+                    const double bc = (tn != 0.0) ? vb[cidx - first] : 0.0;
+#pragma unroll
+                    for (int k = 0; k < TEAM_RW; ++k) {
+                        const int r = rg + k * rstep;
+                        if (r < n) {
+                            double a = A[(size_t)r * n + cidx];
+                            if (tj != 0.0) {
+                                a -= vr[k] * wc + wr[k] * vc;
+                                A[(size_t)r * n + cidx] = a;
+                            }
+                            if (r == first) st_dev(rnext + (cidx - first), a);
+                            acc[k] += a * bc;
+                        }
+                    }
+                }
+                if (tn != 0.0) {
+#pragma unroll
+                    for (int k = 0; k < TEAM_RW; ++k) {
+                        const int r = rg + k * rstep;
+                        const double sacc = wave_sum(acc[k]);
+                        if (r < n && lane == 0) st_dev(pnext + (r - first), tn * sacc);
+                    }
+                }
             }
         }
         ++phase;
@@ -483,23 +534,34 @@ __device__ __forceinline__ void tri_team(const EigMat &mt, int rank, int T, int 
         tj = tn;
     }
     if (rank == 0 && tid == 0) {
-        gdg[n - 1] = ld_dev(rb[n & 1]);  // A[n-1][n-1], published by the last pass
+        gdg[n - 1] = ld_dev((n & 1) ? rb1 : rb0);  // A[n-1][n-1], published by the last pass
         god[n - 1] = 0.0;
     }
 }
 
-// One persistent launch for all matrices of a batch: workgroups of XCD x
-// (blockIdx mod 8) draw tickets from queue x; a job of T tickets is a team
-// of T workgroups (T = 1: the one-workgroup kernel).
+// One persistent launch for the team matrices of a batch.  Every workgroup
+// first draws tickets from the global queue (teams of the largest matrices,
+// spread over XCDs), then workgroups of XCD x (blockIdx mod 8) draw from queue
+// x; a job of T tickets is a team of T workgroups.  Tickets are handed out in
+// job order and a workgroup finishes its job before it draws again, so every
+// team assembles.
+// Matrices below TEAM_MIN_N: one workgroup each.
+__global__ __launch_bounds__(TRI_NT) void k_eig_tri_one(const EigMat *mats, const int *idx) {
+    __shared__ double red[TRI_NW];
+    const EigMat mt = mats[idx[blockIdx.x]];
+    tri_core<true>(mt, red);
+}
+
 __global__ __launch_bounds__(TRI_NT) void k_eig_tri_coop(const EigMat *mats, const TriJob *jobs, const int *tbase,
                                                          const int *qjob, int *counters, int *status) {
     __shared__ double red[TRI_NW];
-    __shared__ double red2[TRI_NT];
     __shared__ int s_ticket, s_flag;
     const int x = blockIdx.x % N_XCD;
-    int *ticket = counters + x;           // counters[0..7]: next ticket per queue
-    int *bars = counters + N_XCD;         // counters[8 + job]: team barrier
-    const int j0 = qjob[x], j1 = qjob[x + 1];
+    int *bars = counters + N_XCD + 1;     // counters[9 + job]: team barrier
+    for (int q = 0; q < 2; ++q) {
+    // queue 0: global (jobs qjob[0]..qjob[1]); queue 1: this XCD's
+    int *ticket = q == 0 ? counters + N_XCD : counters + x;
+    const int j0 = q == 0 ? qjob[0] : qjob[1 + x], j1 = q == 0 ? qjob[1] : qjob[2 + x];
     const int t0 = tbase[j0], total = tbase[j1] - t0;
     for (;;) {
         if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1);
@@ -516,14 +578,10 @@ __global__ __launch_bounds__(TRI_NT) void k_eig_tri_coop(const EigMat *mats, con
         const TriJob jb = jobs[lo];
         const EigMat mt = mats[jb.mat];
         const int rank = t0 + t - tbase[lo];
-        if (jb.T == 1) {
-            if (mt.n <= TRI_LDS_N) tri_core<true>(mt, red);
-            else tri_core<false>(mt, red);
-        } else {
-            if (mt.n <= TRI_LDS_N) tri_team<true>(mt, rank, jb.T, bars + lo, status, red, red2, &s_flag);
-            else tri_team<false>(mt, rank, jb.T, bars + lo, status, red, red2, &s_flag);
-        }
+        if (mt.n <= TEAM_LDS_N) tri_team<true>(mt, rank, jb.T, bars + lo, status, red, &s_flag);
+        else tri_team<false>(mt, rank, jb.T, bars + lo, status, red, &s_flag);
         __syncthreads();
+    }
     }
 }
 
@@ -984,6 +1042,78 @@ __global__ __launch_bounds__(256) void k_eig_bt(const EigMat *mats, const BtItem
     }
 }
 
+// ---------------------------------------------------------------------------
+// 3b. blocked back-transformation for larger matrices (compact WY, GEMMs):
+//     with Q = H_0 H_1 ... H_{n-3}, the eigenvectors (rows of Z) become
+//     Z Q^T = Z H_{n-3} ... H_0.  A block of NB reflectors [j0, j1) is
+//     H_{j0} ... H_{j1-1} = I - Y T Y^T (forward WY, T upper triangular), so
+//     Z <- Z (I - Y T^T Y^T) = Z - (Z M^T) Y^T with M = T Y^T; blocks are
+//     applied from the last to the first.  Y^T (zero-padded reflector rows)
+//     lives in S1; G = Y^T Y, T, M and W = Z M^T in S2.
+// ---------------------------------------------------------------------------
+constexpr int WY_NB = 64;
+constexpr int BT_WY_MIN = 1025;   // register kernel below, WY from here
+constexpr int EIG_MAX_N = 32768;
+
+struct WyItem {
+    double *ws;
+    int n, j0, nb, pad;
+};
+
+// Y^T: row j = reflector j (columns j+1.., first element included), zero
+// elsewhere; rows n-2, n-1 are zero.
+__global__ __launch_bounds__(256) void k_wy_extract(const EigMat *mats, const BtItem *items) {
+    const BtItem it = items[blockIdx.x];
+    const EigMat mt = mats[it.mat];
+    const int n = mt.n;
+    double *Yt = mt.ws + (size_t)n * n;
+    const int j = it.c0;
+    const double *src = mt.A + (size_t)j * n;
+    double *dst = Yt + (size_t)j * n;
+    for (int r = threadIdx.x; r < n; r += 256) dst[r] = (j <= n - 3 && r > j) ? src[r] : 0.0;
+}
+
+// T of the block (dlarft forward / columnwise): T_kk = tau_k,
+// T[0:k, k] = -tau_k T[0:k, 0:k] G[0:k, k], with G = Y^T Y of the block.
+__global__ __launch_bounds__(WY_NB) void k_wy_tfactor(const WyItem *items) {
+    __shared__ double sG[WY_NB][WY_NB + 1];
+    __shared__ double sT[WY_NB][WY_NB + 1];
+    const WyItem it = items[blockIdx.x];
+    const int n = it.n, nb = it.nb, i = threadIdx.x;
+    double *S2 = it.ws + 2 * (size_t)n * n;
+    const double *G = S2;
+    double *T = S2 + WY_NB * WY_NB;
+    const double *tau = vslot(it.ws, n, V_TAU) + it.j0;
+    for (int k = 0; k < nb; ++k) {
+        sG[i][k] = (i < nb) ? G[i * WY_NB + k] : 0.0;
+        sT[i][k] = 0.0;
+    }
+    __syncthreads();
+    for (int k = 0; k < nb; ++k) {
+        const double tk = tau[k];
+        double t = 0.0;
+        if (i < k)
+            for (int l = i; l < k; ++l) t += sT[i][l] * sG[l][k];
+        __syncthreads();
+        if (i < k) sT[i][k] = -tk * t;
+        if (i == k) sT[k][k] = tk;
+        __syncthreads();
+    }
+    if (i < nb)
+        for (int k = 0; k < nb; ++k) T[i * WY_NB + k] = sT[i][k];
+}
+
+// Row c of Z (the back-transformed eigenvector c) -> row rank(c) of A.
+__global__ __launch_bounds__(256) void k_eig_copy_perm(const EigMat *mats, const BtItem *items) {
+    const BtItem it = items[blockIdx.x];
+    const EigMat mt = mats[it.mat];
+    const int n = mt.n, c = it.c0;
+    const int rk = islot(mt.ws, n, V_RANK)[c];
+    const double *src = mt.ws + (size_t)c * n;
+    double *dst = mt.A + (size_t)rk * n;
+    for (int r = threadIdx.x; r < n; r += 256) dst[r] = src[r];
+}
+
 __global__ __launch_bounds__(256) void k_eig_copy(const EigMat *mats, const BtItem *items) {
     const BtItem it = items[blockIdx.x];
     const EigMat mt = mats[it.mat];
@@ -1037,8 +1167,8 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     std::vector<size_t> off(ntasks);
     for (int i = 0; i < ntasks; ++i) {
         if (tasks[i].n <= 0 || !tasks[i].A || !tasks[i].W) return fail(c, NRG_ERR_ARG, "diag: bad task");
-        if (tasks[i].n > 64 * kBtSlots[kBtVariants - 1])
-            return fail(c, NRG_ERR_ARG, "diag: matrix dimension above " + std::to_string(64 * kBtSlots[kBtVariants - 1]));
+        if (tasks[i].n > EIG_MAX_N)
+            return fail(c, NRG_ERR_ARG, "diag: matrix dimension above " + std::to_string(EIG_MAX_N));
         off[i] = tot;
         tot += eig_ws_doubles(tasks[i].n);
     }
@@ -1118,23 +1248,93 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     }
     // back-transformation items by register-slot variant; rank / copy items
     std::vector<BtItem> bt[kBtVariants];
-    std::vector<BtItem> rk, cp;
+    std::vector<BtItem> rk, cp, wyrows;  // wyrows: one row per item (extract, permuted copy)
+    std::vector<int> wymat;              // matrices back-transformed by WY blocks
+    int wy_blocks = 0;
     for (int i = 0; i < nbig; ++i) {
         const int n = tasks[i].n, ns = (n + 63) / 64;
-        int v = 0;
-        while (kBtSlots[v] < ns) ++v;
-        for (int c0 = 0; c0 < n; c0 += 4) bt[v].push_back({i, c0});
+        if (n >= BT_WY_MIN) {
+            wymat.push_back(i);
+            wy_blocks = std::max(wy_blocks, (n - 2 + WY_NB - 1) / WY_NB);
+            for (int r = 0; r < n; ++r) wyrows.push_back({i, r});
+        } else {
+            int v = 0;
+            while (kBtSlots[v] < ns) ++v;
+            for (int c0 = 0; c0 < n; c0 += 4) bt[v].push_back({i, c0});
+            const size_t nn = (size_t)n * n;
+            for (size_t e = 0; e < nn; e += 256 * 16) cp.push_back({i, (int)(e / (256 * 16))});
+        }
         for (int c0 = 0; c0 < n; c0 += 256) rk.push_back({i, c0});
-        const size_t nn = (size_t)n * n;
-        for (size_t e = 0; e < nn; e += 256 * 16) cp.push_back({i, (int)(e / (256 * 16))});
+    }
+    // WY blocks, counted from the last: block q of matrix i covers reflectors
+    // [max(0, j1 - WY_NB), j1) with j1 = n - 2 - q WY_NB
+    std::vector<WyItem> wyitems;
+    std::vector<nrg_gemm_out> wyouts;
+    std::vector<nrg_gemm_term> wyterms;
+    std::vector<GemmItem> wygi;
+    struct WyLevel {
+        int w0, w1;
+        int g[4][2];  // gemm item ranges of G, M, W, Z
+        double flops;
+    };
+    std::vector<WyLevel> wylev(wy_blocks);
+    for (int q = 0; q < wy_blocks; ++q) {
+        WyLevel L{};
+        L.w0 = (int)wyitems.size();
+        std::vector<int> act;
+        for (int i : wymat) {
+            const int n = tasks[i].n, j1 = n - 2 - q * WY_NB;
+            if (j1 <= 0) continue;
+            const int j0 = std::max(0, j1 - WY_NB);
+            wyitems.push_back({c->ews + off[i], n, j0, j1 - j0, 0});
+            act.push_back(i);
+        }
+        L.w1 = (int)wyitems.size();
+        for (int stage = 0; stage < 4; ++stage) {
+            L.g[stage][0] = (int)wygi.size();
+            for (int w = L.w0; w < L.w1; ++w) {
+                const WyItem &it = wyitems[w];
+                const int n = it.n, j0 = it.j0, nb = it.nb, np = n - j0 - 1;
+                double *ws = it.ws, *Yt = ws + (size_t)n * n, *S2 = ws + 2 * (size_t)n * n;
+                double *G = S2, *T = S2 + WY_NB * WY_NB, *M = S2 + 2 * WY_NB * WY_NB, *W = M + (size_t)WY_NB * n;
+                const double *Yb = Yt + (size_t)j0 * n + j0 + 1;  // block rows, columns j0+1..
+                nrg_gemm_out o{};
+                nrg_gemm_term t{};
+                t.alpha = 1.0;
+                if (stage == 0) {  // G = Yb Yb^T
+                    o.C = G; o.m = nb; o.n = nb; o.ldc = WY_NB;
+                    t.A = Yb; t.lda = n; t.B = Yb; t.ldb = n; t.transB = 1; t.k = np;
+                } else if (stage == 1) {  // M = T Yb
+                    o.C = M; o.m = nb; o.n = np; o.ldc = n;
+                    t.A = T; t.lda = WY_NB; t.B = Yb; t.ldb = n; t.k = nb;
+                } else if (stage == 2) {  // W = Z[:, j0+1:] M^T
+                    o.C = W; o.m = n; o.n = nb; o.ldc = WY_NB;
+                    t.A = ws + j0 + 1; t.lda = n; t.B = M; t.ldb = n; t.transB = 1; t.k = np;
+                } else {  // Z[:, j0+1:] -= W Yb
+                    o.C = ws + j0 + 1; o.m = n; o.n = np; o.ldc = n; o.beta = 1.0;
+                    t.A = W; t.lda = WY_NB; t.B = Yb; t.ldb = n; t.k = nb; t.alpha = -1.0;
+                }
+                o.term_begin = (int)wyterms.size();
+                o.term_count = 1;
+                const int g = (int)wyouts.size();
+                wyouts.push_back(o);
+                wyterms.push_back(t);
+                for (int tm = 0; tm * GT < o.m; ++tm)
+                    for (int tn = 0; tn * GT < o.n; ++tn) wygi.push_back({g, tm, tn, 0});
+                if (stage >= 2) L.flops += 2.0 * n * (double)nb * np;
+            }
+            L.g[stage][1] = (int)wygi.size();
+        }
+        wylev[q] = L;
     }
     // tridiagonalisation jobs: ticket ranges, team sizes
     if (c->tri_capacity <= 0) {
         int per_cu = 0, dev = 0, ncu = 0;
         HIPCHK(c, hipGetDevice(&dev));
         HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_coop, TRI_NT,
-                                                               5 * (size_t)TRI_LDS_N * sizeof(double)));
+        HIPCHK(c, hipFuncSetAttribute((const void *)k_eig_tri_coop, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)TRI_COOP_LDS));
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_coop, TRI_NT, TRI_COOP_LDS));
         c->tri_capacity = std::max(1, per_cu * ncu);
     }
     // Team sizes, then longest-processing-time assignment of the jobs to the
@@ -1143,29 +1343,53 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     const int per_xcd = std::max(1, tri_grid / N_XCD);
     std::vector<int> teamT(ntasks, 1);
     std::vector<double> cost(ntasks);
+    std::vector<char> global(ntasks, 0);
     for (int i = 0; i < ntasks; ++i) {
         const double n = tasks[i].n;
+        const int nsl = (tasks[i].n - 1 + TEAM_W - 1) / TEAM_W;
         int T = 1;
-        if (tasks[i].n >= TEAM_MIN_N) T = std::min({(tasks[i].n - 1 + TEAM_W - 1) / TEAM_W, TEAM_MAX, per_xcd});
-        if (T < 2) T = 1;
+        if (tasks[i].n > TEAM_GLOBAL_N) {
+            T = std::min({nsl, TEAM_GLOBAL_MAX, tri_grid});
+            global[i] = 1;
+        } else if (tasks[i].n >= TEAM_MIN_N) {
+            T = std::min({nsl, TEAM_MAX, per_xcd});
+        }
+        if (T < 2) T = 2;  // team jobs only; matrices below TEAM_MIN_N are not queued
         teamT[i] = T;
         // microseconds: one workgroup streams the trailing matrix per step;
         // a team pays a barrier per step plus its share of the matrix
         cost[i] = (T == 1) ? 0.047 * n * n : n * (3.0 + 1.1e-4 * n * n / T);
-        if (T > 1) mats[i].pad = 1;  // reflectors to copy from S2
+        if (tasks[i].n >= TEAM_MIN_N) mats[i].pad = 1;  // reflectors to copy from S2
     }
     std::vector<int> order(ntasks);
     for (int i = 0; i < ntasks; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<int> gq;
     std::vector<std::vector<int>> queue(N_XCD);
     std::vector<double> load(N_XCD, 0.0);
+    for (int i : order)
+        if (global[i]) {
+            gq.push_back(i);
+            for (int x = 0; x < N_XCD; ++x) load[x] += cost[i] * teamT[i] / tri_grid;
+        }
+    std::vector<int> ones;  // one-workgroup matrices
     for (int i : order) {
+        if (global[i]) continue;
+        if (tasks[i].n < TEAM_MIN_N) {
+            ones.push_back(i);
+            continue;
+        }
         const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
         queue[x].push_back(i);
         load[x] += cost[i] * teamT[i] / per_xcd;
     }
     std::vector<TriJob> jobs;
     std::vector<int> tbase(1, 0), qjob(1, 0);
+    for (int i : gq) {
+        jobs.push_back({i, teamT[i]});
+        tbase.push_back(tbase.back() + teamT[i]);
+    }
+    qjob.push_back((int)jobs.size());
     for (int x = 0; x < N_XCD; ++x) {
         for (int i : queue[x]) {
             jobs.push_back({i, teamT[i]});
@@ -1173,7 +1397,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         }
         qjob.push_back((int)jobs.size());
     }
-    const size_t ncnt = N_XCD + (size_t)ntasks;  // ticket counters + one barrier per job
+    const size_t ncnt = N_XCD + 1 + (size_t)ntasks;  // ticket counters + one barrier per job
     if (ncnt > c->cnt_ints) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (c->cnt) HIPCHK(c, hipFree(c->cnt));
@@ -1182,12 +1406,14 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         HIPCHK(c, hipMalloc(&c->cnt, c->cnt_ints * sizeof(int)));
     }
     HostArena ar;
-    const size_t o_jobs = ar.put(jobs), o_tb = ar.put(tbase), o_qj = ar.put(qjob);
+    const size_t o_jobs = ar.put(jobs), o_tb = ar.put(tbase), o_qj = ar.put(qjob), o_ones = ar.put(ones);
     const size_t o_mats = ar.put(mats), o_merges = ar.put(merges), o_sch = ar.put(schunks), o_uch = ar.put(uchunks),
                  o_gouts = ar.put(gouts), o_gterms = ar.put(gterms), o_gitems = ar.put(gitems), o_rk = ar.put(rk),
                  o_cp = ar.put(cp);
     size_t o_bt[kBtVariants];
     for (int v = 0; v < kBtVariants; ++v) o_bt[v] = ar.put(bt[v]);
+    const size_t o_wyrows = ar.put(wyrows), o_wyitems = ar.put(wyitems), o_wyouts = ar.put(wyouts),
+                 o_wyterms = ar.put(wyterms), o_wygi = ar.put(wygi);
     void *d;
     nrg_status s = stage(c, ar.buf.data(), ar.buf.size(), &d);
     if (s) return s;
@@ -1203,7 +1429,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     const GemmItem *d_gitems = (const GemmItem *)(db + o_gitems);
 
     // 1. tridiagonalisation: one persistent launch; matrices of
-    //    TEAM_MIN_N and more get a team of workgroups (one per TEAM_W columns)
+    //    TEAM_MIN_N and more get a team of workgroups (one per TEAM_W rows)
     {
         double fl = 0, by = 0;
         for (int i = 0; i < ntasks; ++i) {
@@ -1215,10 +1441,14 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->dstatus, 1);
         HIPCHK(c, hipGetLastError());
         LaunchTimer lt(K_EIG_TRI, c->stream, fl, by);
+        if (!ones.empty())
+            hipLaunchKernelGGL(k_eig_tri_one, dim3(ones.size()), dim3(TRI_NT), 5 * (size_t)TRI_LDS_N * sizeof(double),
+                               c->stream, d_mats, (const int *)(db + o_ones));
         // grid <= co-resident capacity (one workgroup per CU at this size) and
         // nothing else runs on the stream, so every team assembles; the
         // barrier's bounded wait reports an error otherwise
-        hipLaunchKernelGGL(k_eig_tri_coop, dim3(tri_grid), dim3(TRI_NT), 5 * (size_t)TRI_LDS_N * sizeof(double),
+        if (!jobs.empty())
+            hipLaunchKernelGGL(k_eig_tri_coop, dim3(tri_grid), dim3(TRI_NT), TRI_COOP_LDS,
                            c->stream, d_mats, d_jobs, d_tb, d_qj, c->cnt, c->dstatus);
     }
     HIPCHK(c, hipGetLastError());
@@ -1269,6 +1499,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         double fl = 0, by = 0;
         for (int i = 0; i < nbig; ++i) {
             const double n = tasks[i].n;
+            if (tasks[i].n >= BT_WY_MIN) continue;
             fl += 2.0 * n * n * n;
             by += 8.0 * 2.0 * n * n;
         }
@@ -1278,13 +1509,49 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                 launch_bt_variant(v, c->stream, (int)bt[v].size(), d_mats, (const BtItem *)(db + o_bt[v]));
     }
     HIPCHK(c, hipGetLastError());
-    {
+    if (!cp.empty()) {
         double by = 0;
-        for (int i = 0; i < nbig; ++i) by += 16.0 * tasks[i].n * tasks[i].n;
+        for (int i = 0; i < nbig; ++i)
+            if (tasks[i].n < BT_WY_MIN) by += 16.0 * tasks[i].n * tasks[i].n;
         LaunchTimer lt(K_EIG_COPY, c->stream, 0.0, by);
         hipLaunchKernelGGL(k_eig_copy, dim3(cp.size()), dim3(256), 0, c->stream, d_mats, (const BtItem *)(db + o_cp));
     }
     HIPCHK(c, hipGetLastError());
+    if (!wymat.empty()) {
+        const BtItem *d_wyrows = (const BtItem *)(db + o_wyrows);
+        const WyItem *d_wyitems = (const WyItem *)(db + o_wyitems);
+        const nrg_gemm_out *d_wyouts = (const nrg_gemm_out *)(db + o_wyouts);
+        const nrg_gemm_term *d_wyterms = (const nrg_gemm_term *)(db + o_wyterms);
+        const GemmItem *d_wygi = (const GemmItem *)(db + o_wygi);
+        {
+            LaunchTimer lt(K_EIG_BT, c->stream, 0.0, 0.0);
+            hipLaunchKernelGGL(k_wy_extract, dim3(wyrows.size()), dim3(256), 0, c->stream, d_mats, d_wyrows);
+        }
+        HIPCHK(c, hipGetLastError());
+        for (int q = 0; q < wy_blocks; ++q) {
+            const WyLevel &L = wylev[q];
+            s = gemm_launch_dev(c, K_GEMM_BT, d_wyouts, d_wyterms, d_wygi + L.g[0][0], L.g[0][1] - L.g[0][0], 0.0, 0.0);
+            if (s) return s;
+            {
+                LaunchTimer lt(K_EIG_BT, c->stream, 0.0, 0.0);
+                hipLaunchKernelGGL(k_wy_tfactor, dim3(L.w1 - L.w0), dim3(WY_NB), 0, c->stream, d_wyitems + L.w0);
+            }
+            HIPCHK(c, hipGetLastError());
+            s = gemm_launch_dev(c, K_GEMM_BT, d_wyouts, d_wyterms, d_wygi + L.g[1][0], L.g[1][1] - L.g[1][0], 0.0, 0.0);
+            if (s) return s;
+            s = gemm_launch_dev(c, K_GEMM_BT, d_wyouts, d_wyterms, d_wygi + L.g[2][0], L.g[2][1] - L.g[2][0],
+                                0.5 * L.flops, 0.0);
+            if (s) return s;
+            s = gemm_launch_dev(c, K_GEMM_BT, d_wyouts, d_wyterms, d_wygi + L.g[3][0], L.g[3][1] - L.g[3][0],
+                                0.5 * L.flops, 0.0);
+            if (s) return s;
+        }
+        {
+            LaunchTimer lt(K_EIG_COPY, c->stream, 0.0, 0.0);
+            hipLaunchKernelGGL(k_eig_copy_perm, dim3(wyrows.size()), dim3(256), 0, c->stream, d_mats, d_wyrows);
+        }
+        HIPCHK(c, hipGetLastError());
+    }
     int st = 0;
     HIPCHK(c, hipMemcpyAsync(&st, c->dstatus, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -60,6 +60,7 @@ enum KernelId {
     K_DC_ZHAT,
     K_DC_UNORM,
     K_GEMM_DC,
+    K_GEMM_BT,
     K_DC_RANK,
     K_EIG_BT,
     K_EIG_COPY,

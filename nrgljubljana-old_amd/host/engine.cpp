@@ -674,6 +674,16 @@ struct Engine {
             it.scale = SCALE(N + 1);
             it.arena.reset(new Arena(&gpu));
             determine_tasks(st.prev, it);
+            if (!opt.quiet && !it.subs.empty()) {
+                // sort_task_list's report (src/nrg.cc:2792-2795)
+                int mn = 1 << 30, mx = 0;
+                for (auto &kv : it.subs) {
+                    mn = std::min(mn, kv.second.d);
+                    mx = std::max(mx, kv.second.d);
+                }
+                std::cout << "Stats: nr=" << it.subs.size() << " min=" << mn << " max=" << mx << std::endl;
+            }
+            const auto t_iter = std::chrono::steady_clock::now();
             size_t tot = 0;
             for (auto &kv : it.subs) tot += (size_t)kv.second.d * kv.second.d + kv.second.d;
             it.arena->chunk = std::max<size_t>(tot + 64 * it.subs.size(), 1 << 16);
@@ -805,7 +815,10 @@ struct Engine {
                 nkept += kv.second.kept;
                 nall += kv.second.d;
             }
-            if (!opt.quiet) std::cout << "N=" << N << " subspaces=" << it.subs.size() << " Kept: " << nkept << " out of " << nall << std::endl;
+            if (!opt.quiet)
+                std::cout << "N=" << N << " subspaces=" << it.subs.size() << " Kept: " << nkept << " out of " << nall
+                          << " (" << std::chrono::duration<double>(std::chrono::steady_clock::now() - t_iter).count()
+                          << " s)" << std::endl;
             // ---- recalculation with kept states (strategy=kept)
             st.old_op_arena = std::move(st.op_arena);
             st.op_arena.reset(new Arena(&gpu));

@@ -90,9 +90,23 @@ def test_diag_batch_large(ctx):
         _check_eig(A, W, U, tol=2e-13)
 
 
+def test_diag_batch_beyond_register_limit(ctx):
+    # 2500: a team spread over XCDs; 4200: back-transformation with the
+    # eigenvector partly in LDS
+    rng = np.random.default_rng(7)
+    mats = [_sym(rng, 4200), _sym(rng, 2500), _sym(rng, 300)]
+    out = ctx.diag_batch(mats)
+    for A, (W, U) in zip(mats, out):
+        _check_eig(A, W, U, tol=2e-13)
+
+
 def test_diag_batch_rejects_oversize(ctx):
-    with pytest.raises(RuntimeError):
-        ctx.diag_batch([np.zeros((4097, 4097))])
+    # the size check happens before any device access: a small buffer suffices
+    capi = _capi()
+    t = (capi.EigTask * 1)()
+    t[0].A, t[0].W, t[0].n = ctx.alloc(16), ctx.alloc(16), 32769  # above the documented limit of 32768
+    st = ctx.lib.nrg_diag_batch(ctx.ctx, t, 1)
+    assert st != 0 and b"dimension" in ctx.lib.nrg_last_error(ctx.ctx)
 
 
 def test_diag_batch_many_small_with_shared_spectrum(ctx):
