@@ -26,6 +26,13 @@ sys.path.insert(0, PKG)
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and MFMA)
 PEAK_HBM_GBS = 8000.0
+# What bounds each kernel, and which of its counted quantities is algorithmic:
+# the tridiagonalisation streams the trailing matrix once per column
+# (sum_j 8 (d-j)^2 bytes per block), binning / assembly / transfers are byte
+# streams, the contractions and back-transformations are FP64 arithmetic.
+KERNEL_BOUND = {"k_eig_tri_coop": "hbm", "k_eig_tri_one": "hbm", "k_spec_bin": "hbm", "k_assemble": "hbm", "k_gather_scatter": "hbm",
+                "k_zero": "hbm", "k_eig_copy": "hbm", "k_gemm": "mfma", "k_gemm_dc": "mfma", "k_gemm_bt": "mfma",
+                "k_eig_bt": "mfma"}
 
 CONFIGS = {
     # SIAM QS, keep <= 500 states (Ecut = 8 omega_N), Nmax = 30, DM-NRG spectrum at T=1e-8
@@ -62,9 +69,11 @@ def ensure_tables():
         subprocess.check_call([sys.executable, "-m", "nrgamd.symtab", os.path.join(PKG, "tables")], cwd=PKG)
 
 
-def run_engine(lib, wd, stats):
+def run_engine(lib, wd, stats, plain=False):
     """One solve in-process: chdir into the work directory and call nrg_main."""
     args = [b"nrg", b"--quiet", b"--stats", stats.encode(), b"--tables", os.path.join(PKG, "tables").encode()]
+    if plain:
+        args.append(b"--plain-diag")
     argv = (ctypes.c_char_p * len(args))(*args)
     cwd = os.getcwd()
     os.chdir(wd)
@@ -84,6 +93,36 @@ def run_engine(lib, wd, stats):
         raise RuntimeError("nrg_main returned %d" % rc)
     with open(stats) as f:
         return json.load(f)
+
+
+def serve_measure(nreq, keep_max):
+    """Queued whole solves through the serving loop (continuous batching on
+    this GPU): seeded SIAM QS requests, 4 z-members each, all arriving at t=0,
+    keep drawn from [1000, keep_max]; latency p50/p99 and throughput."""
+    from nrgamd import serve
+    with tempfile.TemporaryDirectory() as out:
+        reqs = serve.make_requests(out, nreq=nreq, seed=1000, rate=None, keep_range=(1000, keep_max))
+        rep = serve.run(out, reqs, slots=16)
+    keys = ["solves", "failed", "makespan_s", "solves_per_s", "eigenpairs_per_s", "latency_p50_s", "latency_p99_s",
+            "request_latency_p50_s", "request_latency_p99_s", "requests_per_s", "shared_launches",
+            "merged_iterations", "slots"]
+    res = {k: rep[k] for k in keys}
+    res["requests"] = nreq
+    res["keep_range"] = [1000, keep_max]
+    return res
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration), or
+    None when no such measurement exists."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        return t[workload][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def cpu_baseline(cfg, budget_s=20.0):
@@ -121,6 +160,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default=DEFAULT, choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the unbatched per-block comparison solve")
+    ap.add_argument("--serve-requests", type=int, default=8,
+                    help="requests (x4 z-members) of the serving measurement; 0 skips it, 64 is the full stream")
+    ap.add_argument("--serve-keep-max", type=int, default=2000)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % a.gpus)
@@ -150,23 +193,43 @@ def main():
                 stage_sum[k] = stage_sum.get(k, 0.0) + v["s"]
         t1 = time.perf_counter()
         kst = capi.kernel_stats(gpulib)
+        gpulib.nrg_timing_enable(0)
+        # the unbatched GPU path on the same workload (one eigensolver call per
+        # block), outside the timed region
+        plain = None
+        if not a.no_plain:
+            print("plain per-block path", file=sys.stderr, flush=True)
+            tp0 = time.perf_counter()
+            stp = run_engine(eng, wd, stats, plain=True)
+            tp = time.perf_counter() - tp0
+            plain = {"value": stp["eigenpairs"] / tp, "ms_per_step": 1e3 * tp, "diag_s": stp["stages"]["diag"]["s"]}
     total = t1 - t0
     value = pairs / total
-    dom = max(kst.items(), key=lambda kv: kv[1]["ms"]) if kst else None
+    if plain:
+        plain["speedup"] = value / plain["value"]
+        plain["diag_speedup"] = plain["diag_s"] / (stage_sum["diag"] / a.steps)
+    timed = {k: v for k, v in kst.items() if k in KERNEL_BOUND and v["ms"] > 0}
+    dom = max(timed.items(), key=lambda kv: kv[1]["ms"]) if timed else None
     roof = None
     if dom:
         name, k = dom
-        if k["flops"] > 0:
+        avg_ms = k["ms"] / max(1, k["launches"])
+        if KERNEL_BOUND[name] == "mfma":
             ach = k["flops"] / (k["ms"] * 1e-3) / 1e12
             roof = {"kernel": name, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": None,
-                    "avg_launch_ms": k["ms"] / max(1, k["launches"])}
+                    "frac": ach / PEAK_FP64_TFLOPS}
         else:
             ach = k["bytes"] / (k["ms"] * 1e-3) / 1e9
             roof = {"kernel": name, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": ach / PEAK_HBM_GBS, "traffic": None,
-                    "avg_launch_ms": k["ms"] / max(1, k["launches"])}
+                    "frac": ach / PEAK_HBM_GBS}
+        roof["traffic"] = measured_traffic(a.config, name)
+        roof["algorithmic_per_launch"] = (k["flops"] if KERNEL_BOUND[name] == "mfma" else k["bytes"]) / max(1, k["launches"])
+        roof["avg_launch_ms"] = avg_ms
     cpu = None if a.no_cpu_baseline else cpu_baseline(cfg)
+    serving = None
+    if a.serve_requests > 0:
+        print("serving %d requests" % a.serve_requests, file=sys.stderr, flush=True)
+        serving = serve_measure(a.serve_requests, a.serve_keep_max)
     out = {
         "metric": "NRG eigenpairs/s (full two-pass solve)",
         "value": value,
@@ -185,6 +248,8 @@ def main():
                    "kernels": {k: {"launches": v["launches"], "ms": v["ms"]} for k, v in kst.items()}},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "plain_per_block": plain,
+        "serving": serving,
     }
     print(json.dumps(out))
 

@@ -62,6 +62,9 @@ nrg_status nrg_h2d(nrg_ctx *ctx, void *dst, const void *src, size_t bytes);
 nrg_status nrg_d2h(nrg_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* Strided device->host gather: `count` doubles at src[k*stride]. */
 nrg_status nrg_d2h_strided(nrg_ctx *ctx, double *dst, const double *src, size_t count, size_t stride);
+/* Free and total device memory of the context's device (bytes); used to admit
+ * solves by HBM footprint.  No counterpart in the reference. */
+nrg_status nrg_mem_info(nrg_ctx *ctx, size_t *free_bytes, size_t *total_bytes);
 /* Fills `count` doubles with zero on the stream (a kernel, not a memset node). */
 nrg_status nrg_zero(nrg_ctx *ctx, double *dst, size_t count);
 

@@ -1428,28 +1428,41 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     nrg_gemm_term *d_gterms = (nrg_gemm_term *)(db + o_gterms);
     const GemmItem *d_gitems = (const GemmItem *)(db + o_gitems);
 
-    // 1. tridiagonalisation: one persistent launch; matrices of
-    //    TEAM_MIN_N and more get a team of workgroups (one per TEAM_W rows)
+    // 1. tridiagonalisation: matrices below TEAM_MIN_N one workgroup each,
+    //    then one persistent launch for the teams
     {
-        double fl = 0, by = 0;
+        double fl1 = 0, by1 = 0, flt = 0, byt = 0;
         for (int i = 0; i < ntasks; ++i) {
             const double n = tasks[i].n;
-            fl += 4.0 / 3.0 * n * n * n;
-            by += 16.0 / 3.0 * n * n * n;  // trailing matrix read + written per step
+            // algorithmic work: 4/3 n^3 flops; the trailing matrix is read once
+            // per column (sum_j 8 (n-j)^2 bytes)
+            (tasks[i].n < TEAM_MIN_N ? fl1 : flt) += 4.0 / 3.0 * n * n * n;
+            (tasks[i].n < TEAM_MIN_N ? by1 : byt) += 8.0 / 3.0 * n * n * n;
         }
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, c->stream, c->cnt, (int)ncnt);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->dstatus, 1);
         HIPCHK(c, hipGetLastError());
-        LaunchTimer lt(K_EIG_TRI, c->stream, fl, by);
-        if (!ones.empty())
-            hipLaunchKernelGGL(k_eig_tri_one, dim3(ones.size()), dim3(TRI_NT), 5 * (size_t)TRI_LDS_N * sizeof(double),
-                               c->stream, d_mats, (const int *)(db + o_ones));
+        // the one-workgroup matrices run beside the teams on the side stream;
+        // they wait for nothing, so team workgroups they delay still start
+        if (!ones.empty()) {
+            HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+            HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+            {
+                LaunchTimer lt(K_EIG_TRI1, c->side, fl1, by1);
+                hipLaunchKernelGGL(k_eig_tri_one, dim3(ones.size()), dim3(TRI_NT),
+                                   5 * (size_t)TRI_LDS_N * sizeof(double), c->side, d_mats, (const int *)(db + o_ones));
+            }
+            HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+        }
         // grid <= co-resident capacity (one workgroup per CU at this size) and
         // nothing else runs on the stream, so every team assembles; the
         // barrier's bounded wait reports an error otherwise
-        if (!jobs.empty())
-            hipLaunchKernelGGL(k_eig_tri_coop, dim3(tri_grid), dim3(TRI_NT), TRI_COOP_LDS,
-                           c->stream, d_mats, d_jobs, d_tb, d_qj, c->cnt, c->dstatus);
+        if (!jobs.empty()) {
+            LaunchTimer lt(K_EIG_TRI, c->stream, flt, byt);
+            hipLaunchKernelGGL(k_eig_tri_coop, dim3(tri_grid), dim3(TRI_NT), TRI_COOP_LDS, c->stream, d_mats, d_jobs,
+                               d_tb, d_qj, c->cnt, c->dstatus);
+        }
+        if (!ones.empty()) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     }
     HIPCHK(c, hipGetLastError());
     if (nbig == 0) return NRG_OK;

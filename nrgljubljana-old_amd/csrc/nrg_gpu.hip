@@ -14,7 +14,7 @@
 // a context-owned device buffer.
 #include "nrg_internal.h"
 
-const char *kKernelNames[K_COUNT] = {"k_zero",       "k_assemble", "k_eig_tri",  "k_dc_leaf", "k_dc_deflate",
+const char *kKernelNames[K_COUNT] = {"k_zero",       "k_assemble", "k_eig_tri_coop", "k_eig_tri_one", "k_dc_leaf", "k_dc_deflate",
                                      "k_dc_secular", "k_dc_zhat",  "k_dc_unorm", "k_gemm_dc", "k_gemm_bt", "k_dc_rank",
                                      "k_eig_bt",     "k_eig_copy", "k_gemm",     "k_spec_bin", "k_gather_scatter"};
 bool g_timing = false;
@@ -337,6 +337,9 @@ nrg_status nrg_ctx_create(nrg_ctx **out, int device) {
     c->device = device;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&c->dstatus, sizeof(int));
     if (e != hipSuccess) {
         fprintf(stderr, "nrg_ctx_create: %s\n", hipGetErrorString(e));
@@ -358,6 +361,10 @@ void nrg_ctx_destroy(nrg_ctx *c) {
     if (c->dstatus) (void)hipFree(c->dstatus);
     if (c->xbuf) (void)hipFree(c->xbuf);
     if (c->hpin) (void)hipHostFree(c->hpin);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -463,6 +470,13 @@ nrg_status nrg_scatter_h2d(nrg_ctx *c, const nrg_strided *items, int nitems, con
     HIPCHK(c, hipGetLastError());
     // the pinned buffer and the staged items are reused by the next call
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return NRG_OK;
+}
+
+nrg_status nrg_mem_info(nrg_ctx *c, size_t *free_bytes, size_t *total_bytes) {
+    if (!c || !free_bytes || !total_bytes) return NRG_ERR_ARG;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemGetInfo(free_bytes, total_bytes));
     return NRG_OK;
 }
 

@@ -16,6 +16,9 @@
 struct nrg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // side stream (default priority) for work that overlaps the main stream
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string err;
     // staging for descriptor arrays
     void *desc = nullptr;
@@ -54,6 +57,7 @@ enum KernelId {
     K_ZERO,
     K_ASSEMBLE,
     K_EIG_TRI,
+    K_EIG_TRI1,
     K_DC_LEAF,
     K_DC_DEFLATE,
     K_DC_SECULAR,

@@ -42,8 +42,9 @@ extern "C" int nrg_main(int argc, char **argv) {
         else if (a == "--tables") opt.tables_dir = next();
         else if (a == "--device") opt.device = std::atoi(next().c_str());
         else if (a == "--quiet") opt.quiet = true;
+        else if (a == "--plain-diag") opt.plain = true;
         else if (a == "-h" || a == "--help") {
-            std::cout << "usage: nrg [-w workdir] [--dump-eigen file] [--stats file] [--tables dir] [--device n] [--quiet]\n";
+            std::cout << "usage: nrg [-w workdir] [--dump-eigen file] [--stats file] [--tables dir] [--device n] [--quiet] [--plain-diag]\n";
             return 0;
         } else nrg::die("unknown option " + a);
     }

@@ -761,7 +761,21 @@ struct Engine {
                     eigenpairs += kv.second.d;
                 }
                 std::stable_sort(tasks.begin(), tasks.end(), [](const nrg_eig_task &a, const nrg_eig_task &b) { return a.n > b.n; });
-                gpu.check(nrg_diag_batch(gpu.ctx, tasks.data(), (int)tasks.size()), "nrg_diag_batch");
+                if (opt.diag) {
+                    // shared launch with the other solves of the process: the
+                    // assembled matrices must be complete before it reads them
+                    gpu.check(nrg_sync(gpu.ctx), "nrg_sync");
+                    if (nrg_status st = opt.diag->diagonalise(tasks.data(), (int)tasks.size()))
+                        die("shared nrg_diag_batch failed (" + std::to_string(st) + ")");
+                } else if (opt.plain) {
+                    // the unbatched path: one eigensolver call per block, in turn
+                    for (auto &t : tasks) {
+                        gpu.check(nrg_diag_batch(gpu.ctx, &t, 1), "nrg_diag_batch");
+                        gpu.check(nrg_sync(gpu.ctx), "nrg_sync");
+                    }
+                } else {
+                    gpu.check(nrg_diag_batch(gpu.ctx, tasks.data(), (int)tasks.size()), "nrg_diag_batch");
+                }
                 std::vector<nrg_strided> items;
                 size_t tot = 0;
                 for (auto &kv : it.subs) {
@@ -1333,13 +1347,16 @@ struct Engine {
 
     int run() {
         auto t0 = std::chrono::steady_clock::now();
-        if (!opt.stats.empty()) {
+        // per-kernel accounting is process-wide: not with concurrent solves
+        if (!opt.stats.empty() && !opt.diag) {
             nrg_timing_enable(1);
             nrg_timing_reset();
         }
         setup();
         if (!opt.dump_eigen.empty()) fdump.open(opt.dump_eigen, std::ios::binary);
+        if (opt.diag) opt.diag->enter();
         pass1();
+        if (opt.diag) opt.diag->leave();
         if (P.dm) {
             dm_sweeps();
             pass2();

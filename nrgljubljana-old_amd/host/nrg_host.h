@@ -139,6 +139,17 @@ struct Iter {
     std::unique_ptr<Arena> arena;  // U, W
 };
 
+// Shared eigensolver for several solves running in one process (the serving
+// loop): a solve announces itself while it is in the NRG pass, and hands each
+// iteration's task list to diagonalise(), which returns when the shared
+// batched launch that contains it has finished.
+struct DiagService {
+    virtual ~DiagService() = default;
+    virtual void enter() = 0;
+    virtual void leave() = 0;
+    virtual nrg_status diagonalise(const nrg_eig_task *tasks, int ntasks) = 0;
+};
+
 struct Options {
     std::string workdir = ".";
     std::string dump_eigen;  // binary dump of eigenvalues / kept counts
@@ -146,6 +157,8 @@ struct Options {
     std::string tables_dir;
     int device = 0;
     bool quiet = false;  // no per-iteration log lines
+    bool plain = false;  // one eigensolver call per block (the unbatched GPU path), for comparison
+    DiagService *diag = nullptr;  // shared eigensolver (serving loop); null: own batched calls
 };
 
 int run(const Options &opt);

@@ -58,6 +58,11 @@ nrg_status nrg_scatter_h2d(nrg_ctx *, const nrg_strided *it, int n, const double
         for (int64_t e = 0; e < it[k].count; ++e) it[k].ptr[e * it[k].stride] = *host++;
     return NRG_OK;
 }
+nrg_status nrg_mem_info(nrg_ctx *, size_t *f, size_t *t) {
+    *f = size_t(1) << 34;
+    *t = size_t(1) << 35;
+    return NRG_OK;
+}
 nrg_status nrg_zero(nrg_ctx *, double *d, size_t n) {
     std::fill(d, d + n, 0.0);
     return NRG_OK;

@@ -61,6 +61,20 @@ def build_mock():
     return out
 
 
+def build_mock_serve():
+    """`nrg-serve` linked against the CPU stand-in of the C ABI (tests only)."""
+    out = os.path.join(REPO, "tests", "mock", "_build", "nrg_serve_mock")
+    srcs = [os.path.join(PKG, "host", f) for f in ("serve_main.cpp", "serve.cpp", "cli.cpp", "engine.cpp", "io.cpp")]
+    srcs.append(os.path.join(REPO, "tests", "mock", "mock_gpu.cpp"))
+    newest = max(os.path.getmtime(s) for s in srcs + [os.path.join(PKG, "host", "nrg_host.h"),
+                                                      os.path.join(REPO, "include", "nrg_gpu.h")])
+    if not os.path.exists(out) or os.path.getmtime(out) < newest:
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        subprocess.check_call(["g++", "-O2", "-std=c++17", "-pthread", "-I" + os.path.join(REPO, "include")] + srcs +
+                              ["-o", out])
+    return out
+
+
 def ensure_tables():
     if not os.path.exists(os.path.join(TABLES, "qs-1ch.tab")):
         subprocess.check_call([sys.executable, "-m", "nrgamd.symtab", TABLES], cwd=PKG)
