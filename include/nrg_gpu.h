@@ -62,6 +62,11 @@ nrg_status nrg_h2d(nrg_ctx *ctx, void *dst, const void *src, size_t bytes);
 nrg_status nrg_d2h(nrg_ctx *ctx, void *dst, const void *src, size_t bytes);
 /* Strided device->host gather: `count` doubles at src[k*stride]. */
 nrg_status nrg_d2h_strided(nrg_ctx *ctx, double *dst, const double *src, size_t count, size_t stride);
+/* Pinned host memory (for spilling device data that does not fit in HBM, as
+ * the reference writes workdir/unitaryN to disk, src/dmnrg.h:198-304);
+ * nrg_h2d / nrg_d2h copy to and from it at full link speed. */
+nrg_status nrg_host_alloc(nrg_ctx *ctx, size_t bytes, void **hptr);
+nrg_status nrg_host_free(nrg_ctx *ctx, void *hptr);
 /* Free and total device memory of the context's device (bytes); used to admit
  * solves by HBM footprint.  No counterpart in the reference. */
 nrg_status nrg_mem_info(nrg_ctx *ctx, size_t *free_bytes, size_t *total_bytes);

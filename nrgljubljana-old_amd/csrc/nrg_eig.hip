@@ -317,6 +317,10 @@ __global__ void k_fill_int(int *p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
+__global__ void k_copy_int(const int *src, int *dst) {
+    if (threadIdx.x == 0) *dst = *src;
+}
+
 // ---------------------------------------------------------------------------
 // 1b. team tridiagonalisation: T workgroups share one matrix; row r of A
 //     belongs to rank r mod T for the whole reduction (cyclic, so the active
@@ -1565,9 +1569,10 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         }
         HIPCHK(c, hipGetLastError());
     }
-    int st = 0;
-    HIPCHK(c, hipMemcpyAsync(&st, c->dstatus, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    // the status word reaches the host through pinned memory (no copy)
+    hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(64), 0, c->stream, c->dstatus, c->hstat);
+    HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (st) return fail(c, NRG_ERR_HIP, "diag: a tridiagonalisation team did not assemble in time");
+    if (*c->hstat) return fail(c, NRG_ERR_HIP, "diag: a tridiagonalisation team did not assemble in time");
     return NRG_OK;
 }

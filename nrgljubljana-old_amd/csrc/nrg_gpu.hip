@@ -40,14 +40,17 @@ nrg_status stage(nrg_ctx *c, const void *host, size_t bytes, void **dev) {
         if (c->desc) {
             HIPCHK(c, hipStreamSynchronize(c->stream));
             HIPCHK(c, hipFree(c->desc));
+            HIPCHK(c, hipHostFree(c->hdesc));
         }
         size_t nb = std::max<size_t>(bytes, 1 << 20);
         HIPCHK(c, hipMalloc(&c->desc, nb));
+        HIPCHK(c, hipHostMalloc(&c->hdesc, nb, hipHostMallocDefault));
         c->desc_bytes = nb;
     }
-    // The staging buffer is reused by the next call: wait for previous users.
+    // The staging buffers are reused by the next call: wait for previous users.
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->desc, host, bytes, hipMemcpyHostToDevice, c->stream));
+    memcpy(c->hdesc, host, bytes);
+    HIPCHK(c, hipMemcpyAsync(c->desc, c->hdesc, bytes, hipMemcpyHostToDevice, c->stream));
     *dev = c->desc;
     return NRG_OK;
 }
@@ -104,14 +107,6 @@ static nrg_status xfer_prepare(nrg_ctx *c, const nrg_strided *items, int nitems,
     *total = (size_t)off;
     *nchunks = (int)xc.size();
     if (!off) return NRG_OK;
-    if ((size_t)off > c->xbuf_doubles) {
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        if (c->xbuf) HIPCHK(c, hipFree(c->xbuf));
-        c->xbuf = nullptr;
-        const size_t nd = std::max<size_t>((size_t)off * 2, 1 << 16);
-        HIPCHK(c, hipMalloc(&c->xbuf, nd * sizeof(double)));
-        c->xbuf_doubles = nd;
-    }
     if ((size_t)off > c->hpin_doubles) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (c->hpin) HIPCHK(c, hipHostFree(c->hpin));
@@ -341,6 +336,7 @@ nrg_status nrg_ctx_create(nrg_ctx **out, int device) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&c->dstatus, sizeof(int));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&c->hstat, sizeof(int), hipHostMallocDefault);
     if (e != hipSuccess) {
         fprintf(stderr, "nrg_ctx_create: %s\n", hipGetErrorString(e));
         delete c;
@@ -359,7 +355,8 @@ void nrg_ctx_destroy(nrg_ctx *c) {
     if (c->ews) (void)hipFree(c->ews);
     if (c->cnt) (void)hipFree(c->cnt);
     if (c->dstatus) (void)hipFree(c->dstatus);
-    if (c->xbuf) (void)hipFree(c->xbuf);
+    if (c->hdesc) (void)hipHostFree(c->hdesc);
+    if (c->hstat) (void)hipHostFree(c->hstat);
     if (c->hpin) (void)hipHostFree(c->hpin);
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -443,11 +440,11 @@ nrg_status nrg_gather_d2h(nrg_ctx *c, const nrg_strided *items, int nitems, doub
     nrg_status s = xfer_prepare(c, items, nitems, &tot, &di, &dc, &nch);
     if (s || !tot) return s;
     {
+        // the kernel writes straight into the pinned host buffer (no copy)
         LaunchTimer lt(K_XFER, c->stream, 0.0, 16.0 * tot);
-        hipLaunchKernelGGL(k_gather, dim3(nch), dim3(256), 0, c->stream, di, dc, c->xbuf);
+        hipLaunchKernelGGL(k_gather, dim3(nch), dim3(256), 0, c->stream, di, dc, c->hpin);
     }
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(c->hpin, c->xbuf, tot * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     memcpy(host, c->hpin, tot * sizeof(double));
     return NRG_OK;
@@ -462,14 +459,27 @@ nrg_status nrg_scatter_h2d(nrg_ctx *c, const nrg_strided *items, int nitems, con
     nrg_status s = xfer_prepare(c, items, nitems, &tot, &di, &dc, &nch);
     if (s || !tot) return s;
     memcpy(c->hpin, host, tot * sizeof(double));
-    HIPCHK(c, hipMemcpyAsync(c->xbuf, c->hpin, tot * sizeof(double), hipMemcpyHostToDevice, c->stream));
     {
+        // the kernel reads straight from the pinned host buffer (no copy)
         LaunchTimer lt(K_XFER, c->stream, 0.0, 16.0 * tot);
-        hipLaunchKernelGGL(k_scatter, dim3(nch), dim3(256), 0, c->stream, di, dc, c->xbuf);
+        hipLaunchKernelGGL(k_scatter, dim3(nch), dim3(256), 0, c->stream, di, dc, c->hpin);
     }
     HIPCHK(c, hipGetLastError());
     // the pinned buffer and the staged items are reused by the next call
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return NRG_OK;
+}
+
+nrg_status nrg_host_alloc(nrg_ctx *c, size_t bytes, void **hptr) {
+    if (!c || !hptr) return NRG_ERR_ARG;
+    hipError_t e = hipHostMalloc(hptr, std::max<size_t>(bytes, 8), hipHostMallocDefault);
+    if (e != hipSuccess) return fail(c, NRG_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    return NRG_OK;
+}
+
+nrg_status nrg_host_free(nrg_ctx *c, void *hptr) {
+    if (!hptr) return NRG_OK;
+    HIPCHK(c, hipHostFree(hptr));
     return NRG_OK;
 }
 

@@ -31,9 +31,10 @@ struct nrg_ctx {
     int *cnt = nullptr;
     size_t cnt_ints = 0;
     int tri_capacity = 0;
-    // batched transfers: device staging and pinned host staging
-    double *xbuf = nullptr;
-    size_t xbuf_doubles = 0;
+    // pinned host staging: descriptors, transfers (read / written by kernels
+    // directly), the eigensolver status
+    void *hdesc = nullptr;
+    int *hstat = nullptr;
     double *hpin = nullptr;
     size_t hpin_doubles = 0;
 };

@@ -43,8 +43,10 @@ extern "C" int nrg_main(int argc, char **argv) {
         else if (a == "--device") opt.device = std::atoi(next().c_str());
         else if (a == "--quiet") opt.quiet = true;
         else if (a == "--plain-diag") opt.plain = true;
+        else if (a == "--hbm-budget") opt.hbm_budget_gb = std::atof(next().c_str());
+        else if (a == "--spill-all") opt.hbm_budget_gb = 0.0;
         else if (a == "-h" || a == "--help") {
-            std::cout << "usage: nrg [-w workdir] [--dump-eigen file] [--stats file] [--tables dir] [--device n] [--quiet] [--plain-diag]\n";
+            std::cout << "usage: nrg [-w workdir] [--dump-eigen file] [--stats file] [--tables dir] [--device n] [--quiet] [--plain-diag] [--hbm-budget GB] [--spill-all]\n";
             return 0;
         } else nrg::die("unknown option " + a);
     }

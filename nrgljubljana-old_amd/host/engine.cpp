@@ -61,6 +61,14 @@ void Gpu::free(void *p) {
     if (it == pool_size.end()) die("Gpu::free: pointer not from the pool");
     pool_free.insert({it->second, it->first});
 }
+void Gpu::trim() {
+    check(nrg_sync(ctx), "nrg_sync");
+    for (auto &kv : pool_free) {
+        check(nrg_dev_free(ctx, kv.second), "nrg_dev_free");
+        pool_size.erase(kv.second);
+    }
+    pool_free.clear();
+}
 void Gpu::gather(const std::vector<nrg_strided> &items, double *host) {
     check(nrg_gather_d2h(ctx, items.data(), (int)items.size(), host), "nrg_gather_d2h");
 }
@@ -159,6 +167,12 @@ struct Engine {
     int Nmax = 0, combs = 0, nch = 0;
     std::vector<Iter> iters;
     double GSenergy = 0.0, totalenergy2 = 0.0, scale_last = 0.0;
+    // eigenvector residency: budget, resident bytes, the order of later uses
+    size_t budget = 0, resident = 0;
+    std::vector<int> schedule;
+    size_t sched_pos = 0;
+    int spills = 0, restores = 0;
+    int pass1_N = -1;  // iteration being built by pass 1 (its predecessor holds E_prev)
     std::map<std::string, Stage> stages;
     long long eigenpairs = 0;
 
@@ -210,6 +224,11 @@ struct Engine {
     void setup() {
         P.load(path("param"));
         D.load(path("data"));
+        {
+            size_t fr = 0, tot = 0;
+            gpu.check(nrg_mem_info(gpu.ctx, &fr, &tot), "nrg_mem_info");
+            budget = opt.hbm_budget_gb >= 0.0 ? (size_t)(opt.hbm_budget_gb * 1e9) : (size_t)(0.6 * (double)fr);
+        }
         nch = D.channels;
         if (P.Nmax == 0) P.Nmax = D.Nmax;
         if ((int)D.xi.at(0).size() - 1 != P.Nmax) die("length of the xi table differs from Nmax");
@@ -641,6 +660,79 @@ struct Engine {
         return cnew;
     }
 
+    // ------------------------------------------------------------ residency
+    // Order in which iterations' eigenvectors are used after pass 1: the
+    // DM-NRG and FDM sweeps go down from the last shell, pass 2 goes up.
+    void plan_schedule() {
+        schedule.clear();
+        sched_pos = 0;
+        if (!P.dm) return;
+        for (int k = 0; k < (P.dmnrg ? 1 : 0) + (P.fdm ? 1 : 0); ++k)
+            for (int N = Nmax - 1; N > P.Ninit; --N) schedule.push_back(N);
+        for (int N = P.Ninit; N < Nmax; ++N) schedule.push_back(N);
+    }
+    size_t next_use(int N) const {
+        for (size_t k = sched_pos; k < schedule.size(); ++k)
+            if (schedule[k] == N) return k;
+        return SIZE_MAX;
+    }
+    void spill(Iter &it) {
+        it.host = nullptr;
+        gpu.check(nrg_host_alloc(gpu.ctx, it.bytes, &it.host), "nrg_host_alloc");
+        gpu.check(nrg_d2h(gpu.ctx, it.host, it.base, it.bytes), "nrg_d2h");
+        it.arena.reset();
+        gpu.trim();
+        it.spilled = true;
+        resident -= it.bytes;
+        ++spills;
+    }
+    void restore(Iter &it) {
+        double *nb = gpu.alloc(it.bytes / 8);
+        gpu.check(nrg_h2d(gpu.ctx, nb, it.host, it.bytes), "nrg_h2d");
+        gpu.check(nrg_host_free(gpu.ctx, it.host), "nrg_host_free");
+        it.host = nullptr;
+        for (auto &kv : it.subs) {
+            kv.second.U = nb + (kv.second.U - it.base);
+            kv.second.W = nb + (kv.second.W - it.base);
+        }
+        it.arena.reset(new Arena(&gpu));
+        it.arena->chunks.push_back(nb);  // released with the arena
+        it.base = nb;
+        it.spilled = false;
+        resident += it.bytes;
+        ++restores;
+    }
+    // Spill resident iterations (the one used last first) until `need` more
+    // bytes fit in the budget; iteration `keep` and, in pass 1, its
+    // predecessor (E_prev) stay.
+    void make_room(size_t need, int keep) {
+        while (resident + need > budget) {
+            int victim = -1;
+            size_t far = 0;
+            for (int N = 0; N < (int)iters.size(); ++N) {
+                const Iter &c = iters[N];
+                if (!c.base || c.spilled || N == keep || (pass1_N >= 0 && N >= pass1_N - 1)) continue;
+                const size_t u = schedule.empty() ? SIZE_MAX - (size_t)N : next_use(N);
+                if (victim < 0 || u > far) {
+                    victim = N;
+                    far = u;
+                }
+            }
+            if (victim < 0) return;  // nothing left to spill: allocate over budget
+            spill(iters[victim]);
+        }
+    }
+    // iteration N is about to be used by a sweep or by pass 2
+    void use(int N) {
+        while (sched_pos < schedule.size() && schedule[sched_pos] != N) ++sched_pos;
+        Iter &it = iters[N];
+        if (it.spilled) {
+            make_room(it.bytes, N);
+            restore(it);
+        }
+        if (sched_pos < schedule.size()) ++sched_pos;
+    }
+
     // ------------------------------------------------------------ pass 1
     void write_dump_iter(const Iter &it) {
         if (!fdump.is_open()) return;
@@ -687,10 +779,16 @@ struct Engine {
             size_t tot = 0;
             for (auto &kv : it.subs) tot += (size_t)kv.second.d * kv.second.d + kv.second.d;
             it.arena->chunk = std::max<size_t>(tot + 64 * it.subs.size(), 1 << 16);
+            it.bytes = 8 * it.arena->chunk;
+            pass1_N = N;
+            make_room(it.bytes, N);
             for (auto &kv : it.subs) {
                 kv.second.U = it.arena->alloc((size_t)kv.second.d * kv.second.d);
                 kv.second.W = it.arena->alloc(kv.second.d);
             }
+            if (it.arena->chunks.size() != 1) die("iteration arena: expected one chunk");
+            it.base = it.arena->chunks[0];
+            resident += it.bytes;
             // ---- assembly + eigensolve
             timed("diag", [&] {
                 std::vector<nrg_asm_block> blocks;
@@ -848,7 +946,9 @@ struct Engine {
             std::map<Invar, Prev> np;
             for (auto &kv : it.subs) np[kv.first] = Prev{kv.second.kept, kv.second.W};
             st.prev = std::move(np);
+            if (N == P.Ninit) plan_schedule();
         }
+        pass1_N = -1;
         GSenergy = totalenergy;
         scale_last = SCALE(Nmax);
         if (!opt.quiet) std::cout << "Total energy: " << std::setprecision(16) << totalenergy << std::endl;
@@ -1024,6 +1124,7 @@ struct Engine {
             timed("DM", [&] {
                 RhoMap cur = rho_init;
                 for (int N = Nmax - 1; N > P.Ninit; --N) {
+                    use(N);
                     Arena tmp(&gpu);
                     RhoMap prev = dm_step(N, cur, nullptr, *rho_arena, tmp);
                     rho_trace_check(prev, 1.0, "DM", N);
@@ -1036,6 +1137,7 @@ struct Engine {
             timed("FDM", [&] {
                 RhoMap cur = init_rho_FDM(Nmax - 1);
                 for (int N = Nmax - 1; N > P.Ninit; --N) {
+                    use(N);
                     Arena tmp(&gpu);
                     const bool last = (N + 1 == Nmax);
                     RhoMap dd;
@@ -1172,7 +1274,7 @@ struct Engine {
             gpu.check(nrg_zero(gpu.ctx, dbins, 2 * (size_t)nbins), "nrg_zero");
             gpu.check(nrg_spec_bin(gpu.ctx, &prm, pairs.data(), (int)pairs.size(), dbins), "nrg_spec_bin");
             std::vector<double> hb(2 * (size_t)nbins);
-            gpu.d2h(hb.data(), dbins, hb.size());
+            gpu.gather({nrg_strided{dbins, (int64_t)hb.size(), 1}}, hb.data());
             std::vector<double> pos(hb.begin(), hb.begin() + nbins), neg(hb.begin() + nbins, hb.end());
             if (fdm) {
                 for (int k = 0; k < nbins; ++k) {
@@ -1192,6 +1294,7 @@ struct Engine {
         const bool recalc_all = P.fdm;  // do_recalc_all (src/nrg.cc:2436-2444)
         for (int N = P.Ninit; N < Nmax; ++N) {
             const bool last = (N + 1 == Nmax);
+            use(N);
             Iter &it = iters[N];
             // the stored values are already shifted by Egs and fixed
             totalenergy += it.Egs * it.scale;
@@ -1327,7 +1430,8 @@ struct Engine {
         if (opt.stats.empty()) return;
         std::ofstream f(opt.stats);
         f << std::setprecision(10) << "{\"wall_s\": " << wall << ", \"eigenpairs\": " << eigenpairs
-          << ", \"Nmax\": " << Nmax << ", \"stages\": {";
+          << ", \"Nmax\": " << Nmax << ", \"spills\": " << spills << ", \"restores\": " << restores
+          << ", \"stages\": {";
         bool first = true;
         for (auto &kv : stages) {
             if (!first) f << ", ";
@@ -1362,6 +1466,8 @@ struct Engine {
             pass2();
             write_spectra();
         }
+        for (auto &it : iters)
+            if (it.host) gpu.check(nrg_host_free(gpu.ctx, it.host), "nrg_host_free");
         iters.clear();
         rhos.clear();
         rhofdm.clear();

@@ -98,6 +98,8 @@ struct Gpu {
     // one transfer for many (strided) device vectors, host side back to back
     void gather(const std::vector<nrg_strided> &items, double *host);
     void scatter(const std::vector<nrg_strided> &items, const double *host);
+    // returns the pool's unused blocks to the device (after a spill)
+    void trim();
 };
 
 // Bump allocator over large device chunks, released together.
@@ -136,7 +138,13 @@ struct Iter {
     int N = 0;
     double scale = 0.0, Egs = 0.0;
     std::map<Invar, Sub> subs;
-    std::unique_ptr<Arena> arena;  // U, W
+    std::unique_ptr<Arena> arena;  // U, W (one chunk)
+    // spill to pinned host memory when the eigenvectors of all iterations do
+    // not fit in the device budget (the reference's workdir/unitaryN files)
+    double *base = nullptr;  // device chunk while resident
+    size_t bytes = 0;
+    void *host = nullptr;    // pinned copy while spilled
+    bool spilled = false;
 };
 
 // Shared eigensolver for several solves running in one process (the serving
@@ -158,6 +166,7 @@ struct Options {
     int device = 0;
     bool quiet = false;  // no per-iteration log lines
     bool plain = false;  // one eigensolver call per block (the unbatched GPU path), for comparison
+    double hbm_budget_gb = -1.0;  // device bytes for resident eigenvectors; <0: 60% of free HBM at start
     DiagService *diag = nullptr;  // shared eigensolver (serving loop); null: own batched calls
 };
 

@@ -58,6 +58,14 @@ nrg_status nrg_scatter_h2d(nrg_ctx *, const nrg_strided *it, int n, const double
         for (int64_t e = 0; e < it[k].count; ++e) it[k].ptr[e * it[k].stride] = *host++;
     return NRG_OK;
 }
+nrg_status nrg_host_alloc(nrg_ctx *, size_t bytes, void **p) {
+    *p = std::malloc(std::max<size_t>(bytes, 8));
+    return *p ? NRG_OK : NRG_ERR_NOMEM;
+}
+nrg_status nrg_host_free(nrg_ctx *, void *p) {
+    std::free(p);
+    return NRG_OK;
+}
 nrg_status nrg_mem_info(nrg_ctx *, size_t *f, size_t *t) {
     *f = size_t(1) << 34;
     *t = size_t(1) << 35;

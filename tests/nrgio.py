@@ -80,11 +80,12 @@ def ensure_tables():
         subprocess.check_call([sys.executable, "-m", "nrgamd.symtab", TABLES], cwd=PKG)
 
 
-def run_engine(binary, workdir, timeout=600):
+def run_engine(binary, workdir, timeout=600, extra=()):
     ensure_tables()
     env = dict(os.environ)
     env["NRG_TABLES"] = TABLES
-    r = subprocess.run([binary, "--dump-eigen", "eig.bin", "--stats", "stats.json"], cwd=workdir, env=env,
+    r = subprocess.run([binary, "--dump-eigen", "eig.bin", "--stats", "stats.json"] + list(extra), cwd=workdir,
+                       env=env,
                        capture_output=True, text=True, timeout=timeout)
     if r.returncode != 0:
         raise RuntimeError("engine failed: %s\n%s" % (r.stdout[-2000:], r.stderr[-2000:]))
