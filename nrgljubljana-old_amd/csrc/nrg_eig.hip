@@ -2,10 +2,12 @@
 // of diagonalise(), src/diag.h:54-89 and 297-353).
 //
 // All matrices of one NRG iteration are solved together, stage by stage:
-//   k_eig_tri    Householder tridiagonalisation, one 1024-thread workgroup per
-//                matrix; step j's rank-2 update is fused with step j+1's
-//                matrix-vector product (one pass over the trailing matrix per
-//                step).  Reflectors stay in the rows of A.
+//   k_eig_tri_*  Householder tridiagonalisation, step j's rank-2 update fused
+//                with step j+1's matrix-vector product.  Up to 128 rows: one
+//                workgroup per matrix, rows in registers (k_eig_tri_one); up
+//                to 576: a team of workgroups with the rows in registers
+//                (k_eig_tri_reg); larger: a team streaming its rows through
+//                L2/HBM (k_eig_tri_coop).  One exchange + barrier per step.
 //   k_dc_*       Cuppen divide and conquer on the tridiagonal matrix: the
 //                tridiagonal is cut into leaves of 1 or 2 rows, then merged
 //                level by level for all matrices at once.  A merge sorts the
@@ -26,9 +28,15 @@
 
 constexpr int TRI_NT = 1024;
 constexpr int TRI_NW = TRI_NT / 64;
-constexpr int TRI_LDS_N = 1344;  // 5 vectors x 1344 x 8 B = 53 KB dynamic LDS
 constexpr int DEFL_NT = 256;
-constexpr int DEFL_LDS_K = 2048;  // 2 x K doubles + 2 x K ints + K doubles = 32 B x K
+constexpr int DEFL_LDS_K = 5000;  // 2 x K doubles + 4 x K ints = 32 B x K of dynamic LDS
+constexpr int DEFL_INLINE_K = 512;  // larger merges gather in column chunks of GATHER_W
+constexpr int GATHER_W = 256;
+// Kind codes of a sorted pole of a merge: bits 0-1 how its row enters the
+// gather (0 deflated by a small z, 1 new pole, 2 rotated into the previous
+// pole); bit 2 set when a later pole rotates it away (deflated); bit 3 marks
+// a rotation candidate of the parallel pass.
+constexpr int KIND_MASK = 3, KIND_SRC = 4, KIND_CAND = 8;
 constexpr int SEC_NT = 64;
 constexpr double kEps = 2.220446049250313e-16;
 
@@ -40,12 +48,10 @@ constexpr double kEps = 2.220446049250313e-16;
 //      eigenvectors of D + rho z z^T
 enum VecSlot {
     V_TAU = 0,  // Householder scalars
-    V_PV,       // p = tau A v
-    V_VA,       // tridiagonalisation vectors (global variant)
-    V_VB,
-    V_WV,
-    V_DGT,
-    V_ODT,
+    V_PV,       // team exchange: p = tau A v (even steps)
+    V_VB,       //   next row (odd steps)
+    V_DGT,      //   p (odd steps)
+    V_ODT,      //   next row (even steps)
     V_DG,    // diagonal of T (after tridiagonalisation)
     V_OD,    // off-diagonal of T
     V_D,     // current eigenvalues (per position)
@@ -58,16 +64,14 @@ enum VecSlot {
     V_META,  // int: [a] = K', [a+1] = number of rotations
     V_RHO,   // rho of the merge at [a]
     V_RANK,  // int: final rank of each eigenvalue
-    V_RP,    // int: rotation pairs (sorted indices)
-    V_RQ,
-    V_RC,
+    V_KIND,  // int per sorted pole: 0 deflated (small z), 1 pole, 2 rotated into the previous pole
+    V_RC,    // Givens c, s of a kind-2 pole (at its sorted index)
     V_RS,
     V_SDS,   // deflation arrays (global variant)
     V_SZS,
     V_PERM,
     V_POS,
-    V_PART,  // TRI_NW vectors of row-group partial sums
-    V_COUNT = V_PART + TRI_NW
+    V_COUNT
 };
 
 __host__ __device__ inline size_t eig_ws_doubles(int n) { return 3 * (size_t)n * n + (size_t)V_COUNT * n; }
@@ -111,207 +115,12 @@ __device__ inline double tri_block_sum(double v, double *red) {
     return s;
 }
 
-// Decomposition of a pass over an (mr x mc) block whose threads own columns:
-// waves take 64-column chunks, and when there are fewer chunks than waves the
-// rows are split into R contiguous groups so every wave keeps loads in flight.
-struct PassMap {
-    int ncc, R, rpg;
-};
-__device__ inline PassMap pass_map(int mr, int mc) {
-    PassMap p;
-    p.ncc = (mc + 63) / 64;
-    p.R = (p.ncc >= TRI_NW) ? 1 : TRI_NW / p.ncc;
-    if (p.R > mr) p.R = mr > 0 ? mr : 1;
-    p.rpg = (mr + p.R - 1) / p.R;
-    return p;
-}
-
 // ---------------------------------------------------------------------------
 // 1. tridiagonalisation: A <- H_j A H_j, j = 0..n-3, H_j = I - tau_j v_j v_j^T
-//    acting on rows/columns j+1..n-1.  v_j is left in row j of A (columns
-//    j+1..n-1, first element included), tau_j in V_TAU, T in V_DG / V_OD.
-// IN_LDS selects at compile time where the work vectors live, so that the
-// LDS variant addresses them with ds_* instructions.
+//    acting on rows/columns j+1..n-1.  v_j ends in row j of A (columns
+//    j+1..n-1, first element included; the streaming teams leave it in S2
+//    for k_dc_leaf to copy), tau_j in V_TAU, T in V_DG / V_OD.
 // ---------------------------------------------------------------------------
-template <bool IN_LDS>
-__device__ __forceinline__ void tri_core(const EigMat &mt, double *red) {
-    extern __shared__ double lds[];
-    const int n = mt.n;
-    double *__restrict__ A = mt.A;
-    const int tid = threadIdx.x;
-    if (n <= 0) return;
-    if (n == 1) {
-        if (tid == 0) {
-            mt.W[0] = A[0];
-            A[0] = 1.0;
-        }
-        return;
-    }
-    double *ws = mt.ws;
-    double *vecs = IN_LDS ? lds : vslot(ws, n, V_VA);
-    double *va = vecs;          // reflector j (length m)
-    double *vb = vecs + n;      // reflector j+1
-    double *wv = vecs + 2 * n;  // w = p - K v
-    double *dg = vecs + 3 * n;
-    double *od = vecs + 4 * n;
-    double *tau = vslot(ws, n, V_TAU);
-    double *pv = vslot(ws, n, V_PV);
-    double *part = vslot(ws, n, V_PART);
-    const int wave = tid >> 6, lane = tid & 63;
-
-    bool p_ready = false;
-    for (int j = 0; j < n - 2; ++j) {
-        const int m = n - j - 1;
-        double *xr = A + (size_t)j * n + j + 1;  // row j right of the diagonal (final)
-        double *A22 = A + (size_t)(j + 1) * n + (j + 1);
-        // reflector of step j (may have been prepared by the previous fused pass)
-        if (!p_ready) {
-            double psum = 0.0;
-            for (int k = 1 + tid; k < m; k += TRI_NT) psum += xr[k] * xr[k];
-            const double sigma = tri_block_sum(psum, red);
-            const double x0 = xr[0];
-            double t = 0.0;
-            if (sigma != 0.0) {
-                const double alpha = -copysign(sqrt(x0 * x0 + sigma), x0);
-                const double v0 = x0 - alpha;
-                t = 2.0 / (v0 * v0 + sigma);
-                for (int k = tid; k < m; k += TRI_NT) va[k] = (k == 0) ? v0 : xr[k];
-                if (tid == 0) od[j] = alpha;
-            } else if (tid == 0) {
-                od[j] = x0;
-            }
-            if (tid == 0) tau[j] = t;
-            __syncthreads();
-            if (t == 0.0) continue;  // nothing to do; next step reads row j+1 as is
-            if (tid == 0) xr[0] = va[0];
-            // p = tau * A22 * v (row groups -> partials -> sum)
-            {
-                const PassMap pm = pass_map(m, m);
-                for (int ch = (pm.ncc >= TRI_NW ? wave : wave % pm.ncc); ch < pm.ncc;
-                     ch += (pm.ncc >= TRI_NW ? TRI_NW : pm.ncc * TRI_NW)) {
-                    const int g = (pm.ncc >= TRI_NW) ? 0 : wave / pm.ncc;
-                    if (g >= pm.R) break;
-                    const int c = ch * 64 + lane;
-                    if (c >= m) continue;
-                    const int r0 = g * pm.rpg, r1 = min(m, r0 + pm.rpg);
-                    double acc = 0.0;
-                    int r = r0;
-                    for (; r + 8 <= r1; r += 8) {
-                        const double *q = A22 + (size_t)r * n + c;
-                        double a[8];
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) a[u] = q[(size_t)u * n];
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) acc += a[u] * va[r + u];
-                    }
-                    for (; r < r1; ++r) acc += A22[(size_t)r * n + c] * va[r];
-                    part[(size_t)g * n + c] = acc;
-                }
-                __syncthreads();
-                for (int c = tid; c < m; c += TRI_NT) {
-                    double acc = 0.0;
-                    for (int g = 0; g < pm.R; ++g) acc += part[(size_t)g * n + c];
-                    pv[c] = t * acc;
-                }
-                __syncthreads();
-            }
-        }
-        const double t = tau[j];
-        // K = tau/2 v.p ; w = p - K v
-        double pp = 0.0;
-        for (int k = tid; k < m; k += TRI_NT) pp += va[k] * pv[k];
-        const double K = 0.5 * t * tri_block_sum(pp, red);
-        for (int k = tid; k < m; k += TRI_NT) wv[k] = pv[k] - K * va[k];
-        __syncthreads();
-        // row 0 of A22 (row j+1 of A) first: it carries the next reflector
-        for (int c = tid; c < m; c += TRI_NT) A22[c] -= va[0] * wv[c] + wv[0] * va[c];
-        __syncthreads();
-        const bool has_next = (j + 1) < n - 2;
-        double tn = 0.0;
-        if (has_next) {
-            const int mn = m - 1;
-            double *xn = A22 + 1;  // row j+1, columns j+2..
-            double psum = 0.0;
-            for (int k = 1 + tid; k < mn; k += TRI_NT) psum += xn[k] * xn[k];
-            const double sigma = tri_block_sum(psum, red);
-            const double x0 = xn[0];
-            if (sigma != 0.0) {
-                const double alpha = -copysign(sqrt(x0 * x0 + sigma), x0);
-                const double v0 = x0 - alpha;
-                tn = 2.0 / (v0 * v0 + sigma);
-                for (int k = tid; k < mn; k += TRI_NT) vb[k] = (k == 0) ? v0 : xn[k];
-                if (tid == 0) od[j + 1] = alpha;
-            } else if (tid == 0) {
-                od[j + 1] = x0;
-            }
-            if (tid == 0) tau[j + 1] = tn;
-            __syncthreads();
-            if (tid == 0 && tn != 0.0) xn[0] = vb[0];
-        }
-        // fused pass over rows/cols 1..m-1 of A22: update, and p' = tn * A33 v'
-        const bool fuse = has_next && tn != 0.0;
-        {
-            const int mm = m - 1;
-            const PassMap pm = pass_map(mm, mm);
-            for (int ch = (pm.ncc >= TRI_NW ? wave : wave % pm.ncc); ch < pm.ncc;
-                 ch += (pm.ncc >= TRI_NW ? TRI_NW : pm.ncc * TRI_NW)) {
-                const int g = (pm.ncc >= TRI_NW) ? 0 : wave / pm.ncc;
-                if (g >= pm.R) break;
-                const int c = 1 + ch * 64 + lane;
-                if (c >= m) continue;
-                const double wc = wv[c], vc = va[c];
-                const int r0 = 1 + g * pm.rpg, r1 = min(m, r0 + pm.rpg);
-                double acc = 0.0;
-                int r = r0;
-                for (; r + 8 <= r1; r += 8) {
-                    double *q = A22 + (size_t)r * n + c;
-                    double a[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) a[u] = q[(size_t)u * n];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        a[u] -= va[r + u] * wc + wv[r + u] * vc;
-                        q[(size_t)u * n] = a[u];
-                        acc += a[u] * vb[r + u - 1];
-                    }
-                }
-                for (; r < r1; ++r) {
-                    double *q = A22 + (size_t)r * n + c;
-                    const double a0 = q[0] - (va[r] * wc + wv[r] * vc);
-                    q[0] = a0;
-                    acc += a0 * vb[r - 1];
-                }
-                if (fuse) part[(size_t)g * n + (c - 1)] = acc;
-            }
-            __syncthreads();
-            if (fuse) {
-                for (int c = tid; c < mm; c += TRI_NT) {
-                    double acc = 0.0;
-                    for (int g = 0; g < pm.R; ++g) acc += part[(size_t)g * n + c];
-                    pv[c] = tn * acc;
-                }
-            }
-        }
-        __syncthreads();
-        if (fuse) {
-            // the next reflector becomes current
-            for (int k = tid; k < m - 1; k += TRI_NT) va[k] = vb[k];
-            __syncthreads();
-        }
-        p_ready = fuse;
-    }
-    if (tid == 0) {
-        od[n - 2] = A[(size_t)(n - 2) * n + n - 1];
-        od[n - 1] = 0.0;
-    }
-    for (int i = tid; i < n; i += TRI_NT) dg[i] = A[(size_t)i * n + i];
-    __syncthreads();
-    double *gdg = vslot(ws, n, V_DG), *god = vslot(ws, n, V_OD);
-    for (int i = tid; i < n; i += TRI_NT) {
-        gdg[i] = dg[i];
-        god[i] = od[i];
-    }
-}
 
 __global__ void k_fill_int(int *p, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
@@ -343,9 +152,7 @@ constexpr int TEAM_W = 16;                   // rows per workgroup when sizing a
 constexpr int TEAM_RW = 2;                   // rows a wavefront updates together
 constexpr int TEAM_CU = 4;                   // column chunks of 64 per round
 constexpr int TEAM_LDS_N = 6800;             // v, v', w of the team in LDS (160 KB)
-constexpr size_t TRI_COOP_LDS = 3 * (size_t)TEAM_LDS_N * sizeof(double);  // >= 5 x TRI_LDS_N doubles
-static_assert(TRI_COOP_LDS >= 5 * (size_t)TRI_LDS_N * sizeof(double), "one-workgroup vectors must fit");
-constexpr int TEAM_MIN_N = 128;              // smaller matrices: one workgroup
+constexpr size_t TRI_COOP_LDS = 3 * (size_t)TEAM_LDS_N * sizeof(double);
 constexpr int TEAM_MAX = 32;                 // workgroups of one XCD
 constexpr int TEAM_GLOBAL_N = 1024;          // larger matrices: teams across XCDs
 constexpr int TEAM_GLOBAL_MAX = 128;
@@ -356,9 +163,15 @@ struct TriJob {
     int mat, T;
 };
 
-__device__ inline void st_dev(double *p, double v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+// (global address space, so that they are global_* rather than flat_*
+// instructions: a scalar base plus a 32-bit lane offset instead of a 64-bit
+// address per element)
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ inline void st_dev(double *p, double v) {
+    __hip_atomic_store((gdouble *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ inline double ld_dev(const double *p) {
-    return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load((gdouble *)const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Team barrier with a bounded wait: returns false (and flags the status) if
@@ -543,17 +356,278 @@ __device__ __forceinline__ void tri_team(const EigMat &mt, int rank, int T, int 
     }
 }
 
+// ---------------------------------------------------------------------------
+// 1c. register-resident reduction for n <= REG_MAX_N.  A team of T
+//     workgroups of NW wavefronts (T = 1: one workgroup, exchange through
+//     LDS) owns the rows of A in registers: row r = rank + T (wave + NW s)
+//     sits in slot s of that wavefront, lane l holding columns l + 64 i.
+//     Teams run in their own persistent kernel of REG_NT threads, whose
+//     register budget (256 per lane) holds RPW x NC doubles of rows plus the
+//     step's vectors without spilling.  Per step j every
+//     wavefront redundantly rebuilds w_j, the updated row j+1 and v_{j+1}
+//     for its own columns from p_j and row j+1 as published by step j-1, so
+//     the only block-level synchronisation of a step is the exchange itself
+//     (one team barrier).  Reductions are wave butterflies, which leave the
+//     same bits in every lane, so all copies of v_{j+1} and tau_{j+1} agree.
+//     Output as tri_core: reflector j in row j of A (columns j+1..), tau,
+//     diagonal and off-diagonal.
+// ---------------------------------------------------------------------------
+constexpr int REG_ONE_N = 128;  // one workgroup: 16 waves x 8 rows, 2 column registers
+constexpr int REG_MAX_N = 576;  // largest team variant: 9 column registers
+constexpr int REG_NT = 512;     // team workgroups: 8 waves
+
+template <int NC>
+__device__ __forceinline__ double reg_at(const double (&x)[NC], int col) {
+    const int i = col >> 6;  // the same in every lane
+    double v = x[0];
+#pragma unroll
+    for (int k = 1; k < NC; ++k)
+        if (i == k) v = x[k];
+    return __shfl(v, col & 63, 64);
+}
+
+// team size of the register variant for dimension n (REG_ONE_N < n <=
+// REG_MAX_N): workgroups of 8 waves x (8, 4, 3) rows
+__host__ __device__ inline int reg_team(int n) {
+    const int rows = (REG_NT / 64) * (n <= 256 ? 8 : n <= 384 ? 4 : 3);
+    return (n + rows - 1) / rows;
+}
+
+// Butterfly sums of R values at once (each leaves the same bits in every
+// lane, as wave_sum); the R shuffle chains are independent and overlap.
+template <int R>
+__device__ __forceinline__ void wave_sum_n(double (&v)[R]) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        double t[R];
+#pragma unroll
+        for (int s = 0; s < R; ++s) t[s] = __shfl_xor(v[s], o, 64);
+#pragma unroll
+        for (int s = 0; s < R; ++s) v[s] += t[s];
+    }
+}
+
+template <int NC, int RPW, int NW, bool TEAM>
+__device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *bar, int *status, int *s_flag,
+                                        double *lbuf) {
+    const int n = mt.n;
+    double *__restrict__ A = mt.A;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (n == 1) {
+        if (rank == 0 && tid == 0) {
+            mt.W[0] = A[0];
+            A[0] = 1.0;
+        }
+        return;
+    }
+    if (n > 64 * NC || T * NW * RPW < n) {  // the same for every workgroup of the team
+        if (tid == 0) atomicCAS(status, 0, NRG_ERR_ARG);
+        return;
+    }
+    double *ws = mt.ws;
+    double *tau = vslot(ws, n, V_TAU), *gdg = vslot(ws, n, V_DG), *god = vslot(ws, n, V_OD);
+    double *const pb0 = TEAM ? vslot(ws, n, V_PV) : lbuf, *const pb1 = TEAM ? vslot(ws, n, V_DGT) : lbuf + n;
+    double *const rb0 = TEAM ? vslot(ws, n, V_ODT) : lbuf + 2 * n, *const rb1 = TEAM ? vslot(ws, n, V_VB) : lbuf + 3 * n;
+    const bool lead = rank == 0 && wave == 0;
+    int row[RPW];
+#pragma unroll
+    for (int s = 0; s < RPW; ++s) row[s] = rank + T * (wave + NW * s);
+    // loads use clamped indices and are masked afterwards, so that they
+    // are unconditional (atomic loads are never speculated)
+    double a[RPW][NC];
+#pragma unroll
+    for (int s = 0; s < RPW; ++s) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            const int c = lane + 64 * i;
+            const double x = A[(size_t)min(row[s], n - 1) * n + min(c, n - 1)];
+            a[s][i] = (row[s] < n && c < n) ? x : 0.0;
+        }
+    }
+    // v_j (columns j+1..), its entries at column j+1 and at the own rows
+    double vv[NC], v1 = 0.0, vrow[RPW];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) vv[i] = 0.0;
+#pragma unroll
+    for (int s = 0; s < RPW; ++s) vrow[s] = 0.0;
+    double tj = 0.0;
+    int phase = 0;
+    for (int j = -1; j <= n - 3; ++j) {
+        double w[NC], xr[NC], w1 = 0.0, wrow[RPW];
+        if (j < 0) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const int c = lane + 64 * i;
+                const double x = A[min(c, n - 1)];
+                xr[i] = c < n ? x : 0.0;
+                w[i] = 0.0;
+            }
+#pragma unroll
+            for (int s = 0; s < RPW; ++s) wrow[s] = 0.0;
+        } else {
+            const double *pg = ((j + 2) & 1) ? pb1 : pb0, *rg = ((j + 2) & 1) ? rb1 : rb0;
+            double pp = 0.0;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                const int c = lane + 64 * i;
+                const bool ok = c > j && c < n;
+                const unsigned cc = (unsigned)min(c, n - 1);
+                const double pl = TEAM ? ld_dev(pg + cc) : pg[cc];
+                const double rl = TEAM ? ld_dev(rg + cc) : rg[cc];
+                w[i] = ok ? pl : 0.0;
+                xr[i] = ok ? rl : 0.0;
+                pp += vv[i] * w[i];
+            }
+            // p at column j+1 and at the own rows, beside the v.p reduction
+            const double p1 = reg_at<NC>(w, j + 1);
+            double prow[RPW];
+#pragma unroll
+            for (int s = 0; s < RPW; ++s) prow[s] = reg_at<NC>(w, row[s]);
+            const double K = 0.5 * tj * wave_sum(pp);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) w[i] -= K * vv[i];
+            w1 = p1 - K * v1;
+#pragma unroll
+            for (int s = 0; s < RPW; ++s) wrow[s] = prow[s] - K * vrow[s];
+#pragma unroll
+            for (int i = 0; i < NC; ++i) xr[i] -= v1 * w[i] + w1 * vv[i];
+            if (lead) {
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    const int c = lane + 64 * i;
+                    if (c > j && c < n) A[(size_t)j * n + c] = vv[i];  // reflector j
+                }
+            }
+        }
+        // row j+1 is final: diagonal x0, and the reflector that zeroes it
+        // beyond column j+2
+        const double x0 = reg_at<NC>(xr, j + 1), x1 = reg_at<NC>(xr, j + 2);
+        double ps = 0.0;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            const int c = lane + 64 * i;
+            if (c > j + 2 && c < n) ps += xr[i] * xr[i];
+        }
+        const double sigma = wave_sum(ps);
+        const bool has_next = j + 1 <= n - 3;
+        double tn = 0.0, alpha = x1, v0 = 0.0;
+        if (has_next && sigma != 0.0) {
+            alpha = -copysign(sqrt(x1 * x1 + sigma), x1);
+            v0 = x1 - alpha;
+            tn = 2.0 / (v0 * v0 + sigma);
+        }
+        const bool live = tn != 0.0;
+        double nv[NC];
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+            const int c = lane + 64 * i;
+            nv[i] = !live ? 0.0 : (c == j + 2) ? v0 : ((c > j + 2 && c < n) ? xr[i] : 0.0);
+        }
+        if (lead && lane == 0) {
+            gdg[j + 1] = x0;
+            god[j + 1] = alpha;
+            if (has_next) tau[j + 1] = tn;
+        }
+        // own rows: update with (v_j, w_j) and p_{j+1}[r] = tn row_r . v_{j+1},
+        // computed for every slot (finished or padding rows are not
+        // published); the owner of row j+2 publishes that row
+        double acc[RPW];
+#pragma unroll
+        for (int s = 0; s < RPW; ++s) {
+            acc[s] = 0.0;
+#pragma unroll
+            for (int i = 0; i < NC; ++i) {
+                a[s][i] -= vrow[s] * w[i] + wrow[s] * vv[i];
+                acc[s] += a[s][i] * nv[i];
+            }
+        }
+        wave_sum_n<RPW>(acc);
+        double *pn = ((j + 3) & 1) ? pb1 : pb0, *rn = ((j + 3) & 1) ? rb1 : rb0;
+#pragma unroll
+        for (int s = 0; s < RPW; ++s) {
+            const int r = row[s];
+            if (r >= j + 2 && r < n) {
+                if (lane == 0) {
+                    if (TEAM) st_dev(pn + r, tn * acc[s]);
+                    else pn[r] = tn * acc[s];
+                }
+                if (r == j + 2) {
+#pragma unroll
+                    for (int i = 0; i < NC; ++i) {
+                        const int c = lane + 64 * i;
+                        if (c >= j + 2 && c < n) {
+                            if (TEAM) st_dev(rn + c, a[s][i]);
+                            else rn[c] = a[s][i];
+                        }
+                    }
+                }
+            }
+        }
+        // v_{j+1} becomes current; its entries at the own rows are gathered
+        // while the stores drain
+#pragma unroll
+        for (int i = 0; i < NC; ++i) vv[i] = nv[i];
+        v1 = live ? v0 : 0.0;
+#pragma unroll
+        for (int s = 0; s < RPW; ++s) vrow[s] = reg_at<NC>(nv, row[s]);
+        tj = tn;
+        if (TEAM) {
+            ++phase;
+            if (!team_barrier(bar, phase * T, status, s_flag)) return;
+        } else {
+            __syncthreads();
+        }
+    }
+    if (rank == 0 && tid == 0) {
+        const double *rl = (n & 1) ? rb1 : rb0;  // row n-1, published by the last step
+        gdg[n - 1] = TEAM ? ld_dev(rl + n - 1) : rl[n - 1];
+        god[n - 1] = 0.0;
+    }
+}
+
 // One persistent launch for the team matrices of a batch.  Every workgroup
 // first draws tickets from the global queue (teams of the largest matrices,
 // spread over XCDs), then workgroups of XCD x (blockIdx mod 8) draw from queue
 // x; a job of T tickets is a team of T workgroups.  Tickets are handed out in
 // job order and a workgroup finishes its job before it draws again, so every
 // team assembles.
-// Matrices below TEAM_MIN_N: one workgroup each.
-__global__ __launch_bounds__(TRI_NT) void k_eig_tri_one(const EigMat *mats, const int *idx) {
-    __shared__ double red[TRI_NW];
+// Matrices up to REG_ONE_N: one workgroup each.
+__global__ __launch_bounds__(TRI_NT) void k_eig_tri_one(const EigMat *mats, const int *idx, int *status) {
+    __shared__ double lbuf[4 * REG_ONE_N];
     const EigMat mt = mats[idx[blockIdx.x]];
-    tri_core<true>(mt, red);
+    tri_reg<2, 8, TRI_NW, false>(mt, 0, 1, nullptr, status, nullptr, lbuf);
+}
+
+// One persistent launch for the register-resident teams: workgroups of XCD
+// x draw tickets from queue x (a team stays on one XCD), as k_eig_tri_coop.
+__global__ __launch_bounds__(REG_NT) void k_eig_tri_reg(const EigMat *mats, const TriJob *jobs, const int *tbase,
+                                                        const int *qjob, int *counters, int *status) {
+    __shared__ int s_ticket, s_flag;
+    const int x = blockIdx.x % N_XCD;
+    int *ticket = counters + x;
+    int *bars = counters + N_XCD + 1;
+    const int j0 = qjob[1 + x], j1 = qjob[2 + x];
+    const int t0 = tbase[j0], total = tbase[j1] - t0;
+    for (;;) {
+        if (threadIdx.x == 0) s_ticket = atomicAdd(ticket, 1);
+        __syncthreads();
+        const int t = s_ticket;
+        __syncthreads();
+        if (t >= total) break;
+        int lo = j0, hi = j1;  // last job of the queue with tbase <= t0 + t
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (tbase[mid] <= t0 + t) lo = mid;
+            else hi = mid;
+        }
+        const TriJob jb = jobs[lo];
+        const EigMat mt = mats[jb.mat];
+        const int rank = t0 + t - tbase[lo];
+        if (mt.n <= 256) tri_reg<4, 8, REG_NT / 64, true>(mt, rank, jb.T, bars + lo, status, &s_flag, nullptr);
+        else if (mt.n <= 384) tri_reg<6, 4, REG_NT / 64, true>(mt, rank, jb.T, bars + lo, status, &s_flag, nullptr);
+        else tri_reg<9, 3, REG_NT / 64, true>(mt, rank, jb.T, bars + lo, status, &s_flag, nullptr);
+        __syncthreads();
+    }
 }
 
 __global__ __launch_bounds__(TRI_NT) void k_eig_tri_coop(const EigMat *mats, const TriJob *jobs, const int *tbase,
@@ -656,119 +730,306 @@ __global__ __launch_bounds__(256) void k_dc_leaf(const EigMat *mats) {
     }
 }
 
-// Sort, z, deflation and gather of one merge [a, b) with boundary mid.
+// Column r of the gathered block of one merge [a, a + K) (h = mid - a), in
+// one pass over the sorted poles q: row pos[q] of G (in S1) is eigenvector
+// perm[q] restricted to its own half, with the deflation rotations applied
+// as they come.  A rotation always pairs pole q with the last pole before it
+// that was not deflated by a small z (the carried value), so the carried
+// element stays in a register until the next pole replaces it.  Rows of the
+// deflated poles (pos >= kp) are final and are copied back into Vc.
+__device__ inline void dc_gather_col(double *ws, int n, int a, int h, int K, int kp, int r, const int *perm,
+                                     const int *pos, const int *kind, const double *rc, const double *rs) {
+    const double *Vc = ws + (size_t)a * n + a;
+    double *G = ws + (size_t)n * n + (size_t)a * n + a;
+    const bool lo = r < h;
+    int cr = -1;
+    double cv = 0.0;
+    constexpr int U = 8;
+    for (int q0 = 0; q0 < K; q0 += U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = q0 + u;
+            v[u] = 0.0;
+            if (q < K) {
+                const int i = perm[q];
+                if ((i < h) == lo) v[u] = Vc[(size_t)i * n + r];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = q0 + u;
+            if (q >= K) break;
+            const int k = kind[q] & KIND_MASK;
+            if (k == 0) {
+                G[(size_t)pos[q] * n + r] = v[u];
+            } else if (k == 1) {
+                if (cr >= 0) G[(size_t)pos[cr] * n + r] = cv;
+                cr = q;
+                cv = v[u];
+            } else {
+                const double c = rc[q], s = rs[q];
+                G[(size_t)pos[cr] * n + r] = c * cv + s * v[u];
+                cv = c * v[u] - s * cv;
+                cr = q;
+            }
+        }
+    }
+    if (cr >= 0) G[(size_t)pos[cr] * n + r] = cv;
+    double *Vw = ws + (size_t)a * n + a;
+    for (int p0 = kp; p0 < K; p0 += U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (p0 + u < K) v[u] = G[(size_t)(p0 + u) * n + r];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (p0 + u < K) Vw[(size_t)(p0 + u) * n + r] = v[u];
+    }
+}
+
+// Ascending sort of (key[i], idx[i]), i < K, by the total order (key, idx):
+// the flip form of Batcher's bitonic network, in which every comparator puts
+// the smaller element at the lower index.  Padding up to the power of two P
+// is conceptually +inf and never moves, so comparators reaching j >= K are
+// skipped.  key/idx are in LDS or in global memory (one workgroup).
+__device__ inline void dc_bitonic_sort(double *key, int *idx, int K, int NT) {
+    int P = 1;
+    while (P < K) P <<= 1;
+    const int tid = threadIdx.x;
+    for (int B = 2; B <= P; B <<= 1) {
+        for (int d = B >> 1; d >= 1; d >>= 1) {
+            const bool flip = d == (B >> 1);
+            for (int c = tid; c < (P >> 1); c += NT) {
+                int i, j;
+                if (flip) {
+                    const int blk = c / d, t = c - blk * d;
+                    i = blk * B + t;
+                    j = blk * B + B - 1 - t;
+                } else {
+                    const int blk = c / d, t = c - blk * d;
+                    i = blk * 2 * d + t;
+                    j = i + d;
+                }
+                if (j >= K) continue;
+                const double ki = key[i], kj = key[j];
+                const int ii = idx[i], ij = idx[j];
+                if (kj < ki || (kj == ki && ij < ii)) {
+                    key[i] = kj;
+                    key[j] = ki;
+                    idx[i] = ij;
+                    idx[j] = ii;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Block-wide exclusive scans of one int per thread (sum, or max with
+// identity -1); every thread also receives the total.
+template <bool MAX>
+__device__ inline int dc_block_scan(int x, int *sred, int &total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc = MAX ? max(inc, y) : inc + y;
+    }
+    __syncthreads();
+    if (lane == 63) sred[w] = inc;
+    __syncthreads();
+    int base = MAX ? -1 : 0, all = MAX ? -1 : 0;
+#pragma unroll
+    for (int k = 0; k < DEFL_NT / 64; ++k) {
+        const int v = sred[k];
+        if (k < w) base = MAX ? max(base, v) : base + v;
+        all = MAX ? max(all, v) : all + v;
+    }
+    total = all;
+    int ex = __shfl_up(inc, 1, 64);
+    if (lane == 0) ex = MAX ? -1 : 0;
+    return MAX ? max(base, ex) : base + ex;
+}
+
+// Deflation test of sorted pole q against the previous undeflated pole p
+// (the same arithmetic for the parallel first pass and for rotation chains).
+__device__ inline bool dc_close(double zp, double zq, double dp, double dq, double tol, double &c, double &s,
+                                double &tau) {
+    s = zp;
+    c = zq;
+    tau = hypot(c, s);
+    const double t = dq - dp;
+    c /= tau;
+    s = -s / tau;
+    return fabs(t * c * s) <= tol;
+}
+
+// Sort, z and deflation of one merge [a, b) with boundary mid; merges up to
+// DEFL_INLINE_K also gather here (inline_gather), larger ones in k_dc_gather.
+// Deflation follows the sequential rule of LAPACK dlaed2: a pole with a
+// small z component is deflated; otherwise, when it is close to the previous
+// undeflated pole, a Givens rotation moves that pole's z into it and the
+// previous pole is deflated.  The test of pole q depends on earlier
+// decisions only through a rotation into its predecessor, so all tests are
+// made in parallel and one thread re-walks only candidates and the poles
+// that follow a rotation.
 template <bool IN_LDS>
 __global__ __launch_bounds__(DEFL_NT) void k_dc_deflate(const DcMerge *merges, nrg_gemm_out *outs,
-                                                      nrg_gemm_term *terms) {
+                                                      nrg_gemm_term *terms, int inline_gather) {
     extern __shared__ double lds[];
     __shared__ double red[DEFL_NT / 64];
+    __shared__ int sred[DEFL_NT / 64];
     __shared__ int s_info[2];
     const DcMerge mg = merges[blockIdx.x];
     const int n = mg.n, a = mg.a, h = mg.mid - mg.a, K = mg.b - mg.a;
     double *ws = mg.ws;
-    double *Vc = ws, *S1 = ws + (size_t)n * n;
+    const double *Vc = ws;
     double *D = vslot(ws, n, V_D) + a;
     const double beta = vslot(ws, n, V_BETA)[mg.mid];
     const double rho = 2.0 * fabs(beta);
     const double sg = beta >= 0.0 ? 1.0 : -1.0;
     const double rs2 = 0.70710678118654752440;
     const int tid = threadIdx.x;
-    double *draw, *ds, *zs;
-    int *perm, *pos;
+    double *ds, *zs;
+    int *perm, *pos, *prev, *cand;
     if (IN_LDS) {
-        draw = lds;
-        ds = lds + K;
-        zs = lds + 2 * K;
-        perm = reinterpret_cast<int *>(lds + 3 * K);
+        ds = lds;
+        zs = lds + K;
+        perm = reinterpret_cast<int *>(lds + 2 * K);
         pos = perm + K;
+        prev = pos + K;
+        cand = prev + K;
     } else {
-        draw = vslot(ws, n, V_TS) + a;  // free until the secular stage
         ds = vslot(ws, n, V_SDS) + a;
         zs = vslot(ws, n, V_SZS) + a;
         perm = islot(ws, n, V_PERM) + a;
         pos = islot(ws, n, V_POS) + a;
+        prev = islot(ws, n, V_ORG) + a;  // free until the secular stage
+        cand = islot(ws, n, V_TS) + a;
     }
-    for (int i = tid; i < K; i += DEFL_NT) draw[i] = D[i];
-    __syncthreads();
-    // stable ascending rank of the poles
+    int *kind = islot(ws, n, V_KIND) + a;
+    double *rc = vslot(ws, n, V_RC) + a, *rsn = vslot(ws, n, V_RS) + a;
     for (int i = tid; i < K; i += DEFL_NT) {
-        const double di = draw[i];
-        int rk = 0;
-        for (int j = 0; j < K; ++j) {
-            const double dj = draw[j];
-            rk += (dj < di) || (dj == di && j < i);
-        }
-        perm[rk] = i;
+        ds[i] = D[i];
+        perm[i] = i;
     }
     __syncthreads();
+    dc_bitonic_sort(ds, perm, K, DEFL_NT);
     double md = 0.0, mz = 0.0;
     for (int q = tid; q < K; q += DEFL_NT) {
         const int i = perm[q];
         const size_t c = (size_t)(a + i);
         const double z = (i < h) ? Vc[c * n + (mg.mid - 1)] * rs2 : sg * Vc[c * n + mg.mid] * rs2;
-        ds[q] = draw[i];
         zs[q] = z;
-        md = fmax(md, fabs(draw[i]));
+        md = fmax(md, fabs(ds[q]));
         mz = fmax(mz, fabs(z));
     }
     md = block_max<DEFL_NT>(md, red);
     mz = block_max<DEFL_NT>(mz, red);
     const double tol = 8.0 * kEps * fmax(md, rho * mz);
-    int *rp = islot(ws, n, V_RP) + a, *rq = islot(ws, n, V_RQ) + a;
-    double *rc = vslot(ws, n, V_RC) + a, *rsn = vslot(ws, n, V_RS) + a;
-    if (tid == 0) {
-        // pos[q] = -1 marks a deflated pole
-        int pj = -1, nrot = 0;
-        for (int q = 0; q < K; ++q) {
-            pos[q] = 0;
-            if (rho * fabs(zs[q]) <= tol) {
-                pos[q] = -1;
-                continue;
-            }
-            if (pj < 0) {
-                pj = q;
-                continue;
-            }
-            double s = zs[pj], c = zs[q];
-            const double tau = hypot(c, s);
-            const double t = ds[q] - ds[pj];
-            c /= tau;
-            s = -s / tau;
-            if (fabs(t * c * s) <= tol) {
-                // close poles: rotate the pair so that pj's z component vanishes
-                zs[q] = tau;
-                zs[pj] = 0.0;
-                rp[nrot] = pj;
-                rq[nrot] = q;
-                rc[nrot] = c;
-                rsn[nrot] = s;
-                ++nrot;
-                const double tt = ds[pj] * c * c + ds[q] * s * s;
-                ds[q] = ds[pj] * s * s + ds[q] * c * c;
-                ds[pj] = tt;
-                pos[pj] = -1;
-            }
-            pj = q;
+    // previous pole with a large z: exclusive max-scan over contiguous
+    // segments of the sorted order
+    const int per = (K + DEFL_NT - 1) / DEFL_NT, q0 = min(K, tid * per), q1 = min(K, q0 + per);
+    {
+        int m = -1;
+        for (int q = q0; q < q1; ++q)
+            if (!(rho * fabs(zs[q]) <= tol)) m = q;
+        int tot;
+        int run = dc_block_scan<true>(m, sred, tot);
+        for (int q = q0; q < q1; ++q) {
+            prev[q] = run;
+            if (!(rho * fabs(zs[q]) <= tol)) run = q;
         }
-        // positions: non-deflated first (ascending, as they stay sorted),
-        // then the deflated ones
-        int kp = 0;
-        for (int q = 0; q < K; ++q)
-            if (pos[q] >= 0) pos[q] = kp++;
-        int kd = kp;
-        for (int q = 0; q < K; ++q)
-            if (pos[q] < 0) pos[q] = kd++;
-        s_info[0] = kp;
-        s_info[1] = nrot;
-        int *meta = islot(ws, n, V_META) + a;
-        meta[0] = kp;
-        meta[1] = nrot;
-        vslot(ws, n, V_RHO)[a] = rho;
-        outs[mg.gemm].m = kp;
-        terms[mg.gemm].k = kp;
     }
     __syncthreads();
-    const int kp = s_info[0], nrot = s_info[1];
+    // parallel tests with the unrotated values
+    int ncand_local = 0;
+    for (int q = q0; q < q1; ++q) {
+        int k = 0;
+        if (!(rho * fabs(zs[q]) <= tol)) {
+            k = 1;
+            const int p = prev[q];
+            double c, s, tau;
+            if (p >= 0 && dc_close(zs[p], zs[q], ds[p], ds[q], tol, c, s, tau)) {
+                k |= KIND_CAND;
+                ++ncand_local;
+            }
+        }
+        kind[q] = k;
+    }
+    int ncand;
+    int cbase = dc_block_scan<false>(ncand_local, sred, ncand);
+    for (int q = q0; q < q1; ++q)
+        if (kind[q] & KIND_CAND) cand[cbase++] = q;
+    __syncthreads();
+    if (tid == 0) {
+        int ci = 0, chain = -1, nrot = 0;
+        for (;;) {
+            int q;
+            if (chain >= 0) {
+                q = chain;
+                chain = -1;
+                while (ci < ncand && cand[ci] <= q) ++ci;
+            } else if (ci < ncand) {
+                q = cand[ci++];
+            } else {
+                break;
+            }
+            const int p = prev[q];
+            double c, s, tau;
+            int k = kind[q] & ~KIND_CAND;
+            if (dc_close(zs[p], zs[q], ds[p], ds[q], tol, c, s, tau)) {
+                zs[q] = tau;
+                zs[p] = 0.0;
+                k = (k & ~KIND_MASK) | 2;
+                rc[q] = c;
+                rsn[q] = s;
+                ++nrot;
+                const double tt = ds[p] * c * c + ds[q] * s * s;
+                ds[q] = ds[p] * s * s + ds[q] * c * c;
+                ds[p] = tt;
+                kind[p] |= KIND_SRC;
+                // the next large-z pole is tested against the rotated values
+                int nx = q + 1;
+                while (nx < K && rho * fabs(zs[nx]) <= tol) ++nx;
+                if (nx < K) chain = nx;
+            }
+            kind[q] = k;
+        }
+        s_info[1] = nrot;
+    }
+    __syncthreads();
+    // positions: undeflated poles first (ascending, as they stay sorted),
+    // then the deflated ones in sorted order
+    {
+        int und = 0;
+        for (int q = q0; q < q1; ++q) {
+            const int k = kind[q];
+            und += (k & KIND_MASK) != 0 && !(k & KIND_SRC);
+        }
+        int kp;
+        int ub = dc_block_scan<false>(und, sred, kp);
+        int db = kp + (q0 - (ub));  // deflated before q0 = q0 - undeflated before q0
+        for (int q = q0; q < q1; ++q) {
+            const int k = kind[q] & ~KIND_CAND;
+            kind[q] = k;
+            if ((k & KIND_MASK) != 0 && !(k & KIND_SRC)) pos[q] = ub++;
+            else pos[q] = db++;
+        }
+        if (tid == 0) {
+            s_info[0] = kp;
+            int *meta = islot(ws, n, V_META) + a;
+            meta[0] = kp;
+            meta[1] = s_info[1];
+            vslot(ws, n, V_RHO)[a] = rho;
+            outs[mg.gemm].m = kp;
+            terms[mg.gemm].k = kp;
+        }
+    }
+    __syncthreads();
+    const int kp = s_info[0];
     double *dd = vslot(ws, n, V_DD) + a, *zz = vslot(ws, n, V_ZZ) + a;
     for (int q = tid; q < K; q += DEFL_NT) {
         const int p = pos[q];
@@ -779,38 +1040,29 @@ __global__ __launch_bounds__(DEFL_NT) void k_dc_deflate(const DcMerge *merges, n
             D[p] = ds[q];  // deflated eigenvalue, final for this merge
         }
     }
-    // gather: row pos(q) of the block of S1 = eigenvector perm[q] restricted to
-    // its own half (zero in the other half)
-    double *G = S1 + (size_t)a * n + a;
-    const int wave = tid >> 6, lane = tid & 63;
-    for (int q = wave; q < K; q += DEFL_NT / 64) {
-        const int i = perm[q];
-        const double *src = Vc + (size_t)(a + i) * n + a;
-        double *dst = G + (size_t)pos[q] * n;
-        const int r0 = (i < h) ? 0 : h, r1 = (i < h) ? h : K;
-        for (int r = lane; r < K; r += 64) dst[r] = (r >= r0 && r < r1) ? src[r] : 0.0;
-    }
-    __syncthreads();
-    // deflation rotations, in order; each thread owns element r of every row
-    if (nrot > 0) {
-        for (int r = tid; r < K; r += DEFL_NT) {
-            for (int t = 0; t < nrot; ++t) {
-                double *x = G + (size_t)pos[rp[t]] * n + r;
-                double *y = G + (size_t)pos[rq[t]] * n + r;
-                const double c = rc[t], s = rsn[t];
-                const double xv = *x, yv = *y;
-                *x = c * xv + s * yv;
-                *y = c * yv - s * xv;
-            }
+    if (inline_gather) {
+        for (int r = tid; r < K; r += DEFL_NT) dc_gather_col(ws, n, a, h, K, kp, r, perm, pos, kind, rc, rsn);
+    } else if (IN_LDS) {
+        // the column-chunk gather reads the permutation from global memory
+        int *gperm = islot(ws, n, V_PERM) + a, *gpos = islot(ws, n, V_POS) + a;
+        for (int q = tid; q < K; q += DEFL_NT) {
+            gperm[q] = perm[q];
+            gpos[q] = pos[q];
         }
     }
-    __syncthreads();
-    // deflated eigenvectors are final for this merge
-    for (int p = kp + wave; p < K; p += DEFL_NT / 64) {
-        const double *src = G + (size_t)p * n;
-        double *dst = Vc + (size_t)(a + p) * n + a;
-        for (int r = lane; r < K; r += 64) dst[r] = src[r];
-    }
+}
+
+// Gather of columns [c0, c0 + GATHER_W) of one merge, one column per thread
+// (the deflation rotations mix rows only, so columns are independent).
+__global__ __launch_bounds__(GATHER_W) void k_dc_gather(const DcMerge *merges, const DcChunk *chunks) {
+    const DcChunk ck = chunks[blockIdx.x];
+    const DcMerge mg = merges[ck.merge];
+    const int n = mg.n, a = mg.a, K = mg.b - mg.a;
+    const int r = ck.j0 + threadIdx.x;
+    if (r >= K) return;
+    double *ws = mg.ws;
+    dc_gather_col(ws, n, a, mg.mid - a, K, islot(ws, n, V_META)[a], r, islot(ws, n, V_PERM) + a,
+                  islot(ws, n, V_POS) + a, islot(ws, n, V_KIND) + a, vslot(ws, n, V_RC) + a, vslot(ws, n, V_RS) + a);
 }
 
 // Root j of 1/rho + sum_i zz_i^2 / (dd_i - lambda) = 0 (dd ascending,
@@ -1194,12 +1446,12 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     int maxlev = 0;
     for (int i = 0; i < nbig; ++i) maxlev = std::max(maxlev, dc_levels(tasks[i].n));
     std::vector<DcMerge> merges;
-    std::vector<DcChunk> schunks, uchunks;
+    std::vector<DcChunk> schunks, uchunks, xchunks;
     std::vector<nrg_gemm_out> gouts;
     std::vector<nrg_gemm_term> gterms;
     std::vector<GemmItem> gitems;
     struct Level {
-        int m0, m1, s0, s1, u0, u1, g0, g1, maxK;
+        int m0, m1, s0, s1, u0, u1, g0, g1, x0, x1, maxK;
         double flops;
     };
     std::vector<Level> levels(maxlev + 1);
@@ -1209,6 +1461,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         lv.s0 = (int)schunks.size();
         lv.u0 = (int)uchunks.size();
         lv.g0 = (int)gitems.size();
+        lv.x0 = (int)xchunks.size();
         for (int i = 0; i < nbig; ++i) {
             const int n = tasks[i].n, nlev = dc_levels(n);
             if (L > nlev) continue;
@@ -1248,6 +1501,12 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         lv.s1 = (int)schunks.size();
         lv.u1 = (int)uchunks.size();
         lv.g1 = (int)gitems.size();
+        // a level with a merge above DEFL_INLINE_K gathers all its merges in
+        // column chunks; otherwise each deflation workgroup gathers its own
+        if (lv.maxK > DEFL_INLINE_K)
+            for (int mi = lv.m0; mi < lv.m1; ++mi)
+                for (int j0 = 0; j0 < merges[mi].b - merges[mi].a; j0 += GATHER_W) xchunks.push_back({mi, j0});
+        lv.x1 = (int)xchunks.size();
         levels[L] = lv;
     }
     // back-transformation items by register-slot variant; rank / copy items
@@ -1340,37 +1599,50 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                                       (int)TRI_COOP_LDS));
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_coop, TRI_NT, TRI_COOP_LDS));
         c->tri_capacity = std::max(1, per_cu * ncu);
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_reg, REG_NT, 0));
+        c->reg_capacity = std::max(1, per_cu * ncu);
     }
     // Team sizes, then longest-processing-time assignment of the jobs to the
     // XCD queues (work = estimated time x workgroups; a team stays on one XCD)
     const int tri_grid = c->tri_capacity;
     const int per_xcd = std::max(1, tri_grid / N_XCD);
+    const int reg_grid = c->reg_capacity, reg_per_xcd = std::max(1, reg_grid / N_XCD);
     std::vector<int> teamT(ntasks, 1);
     std::vector<double> cost(ntasks);
     std::vector<char> global(ntasks, 0);
+    std::vector<char> regjob(ntasks, 0);
     for (int i = 0; i < ntasks; ++i) {
         const double n = tasks[i].n;
+        if (tasks[i].n <= REG_ONE_N) {  // one workgroup on the side stream, not queued
+            cost[i] = 0.5 * n;
+            continue;
+        }
+        // microseconds per step: a register-resident team pays its exchange;
+        // a streaming team a barrier plus its share of the matrix
+        if (tasks[i].n <= REG_MAX_N && reg_team(tasks[i].n) <= reg_per_xcd) {
+            teamT[i] = reg_team(tasks[i].n);
+            regjob[i] = 1;
+            cost[i] = 2.0 * n;
+            continue;
+        }
         const int nsl = (tasks[i].n - 1 + TEAM_W - 1) / TEAM_W;
-        int T = 1;
+        int T;
         if (tasks[i].n > TEAM_GLOBAL_N) {
             T = std::min({nsl, TEAM_GLOBAL_MAX, tri_grid});
             global[i] = 1;
-        } else if (tasks[i].n >= TEAM_MIN_N) {
+        } else {
             T = std::min({nsl, TEAM_MAX, per_xcd});
         }
-        if (T < 2) T = 2;  // team jobs only; matrices below TEAM_MIN_N are not queued
-        teamT[i] = T;
-        // microseconds: one workgroup streams the trailing matrix per step;
-        // a team pays a barrier per step plus its share of the matrix
-        cost[i] = (T == 1) ? 0.047 * n * n : n * (3.0 + 1.1e-4 * n * n / T);
-        if (tasks[i].n >= TEAM_MIN_N) mats[i].pad = 1;  // reflectors to copy from S2
+        teamT[i] = std::max(T, 2);
+        cost[i] = n * (3.0 + 1.1e-4 * n * n / teamT[i]);
+        mats[i].pad = 1;  // reflectors to copy from S2
     }
     std::vector<int> order(ntasks);
     for (int i = 0; i < ntasks; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
     std::vector<int> gq;
-    std::vector<std::vector<int>> queue(N_XCD);
-    std::vector<double> load(N_XCD, 0.0);
+    std::vector<std::vector<int>> queue(N_XCD), rqueue(N_XCD);
+    std::vector<double> load(N_XCD, 0.0), rload(N_XCD, 0.0);
     for (int i : order)
         if (global[i]) {
             gq.push_back(i);
@@ -1379,29 +1651,39 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     std::vector<int> ones;  // one-workgroup matrices
     for (int i : order) {
         if (global[i]) continue;
-        if (tasks[i].n < TEAM_MIN_N) {
+        if (tasks[i].n <= REG_ONE_N) {
             ones.push_back(i);
             continue;
         }
-        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        queue[x].push_back(i);
-        load[x] += cost[i] * teamT[i] / per_xcd;
+        std::vector<double> &ld = regjob[i] ? rload : load;
+        const int x = (int)(std::min_element(ld.begin(), ld.end()) - ld.begin());
+        (regjob[i] ? rqueue : queue)[x].push_back(i);
+        ld[x] += cost[i] * teamT[i] / (regjob[i] ? reg_per_xcd : per_xcd);
     }
-    std::vector<TriJob> jobs;
-    std::vector<int> tbase(1, 0), qjob(1, 0);
-    for (int i : gq) {
-        jobs.push_back({i, teamT[i]});
-        tbase.push_back(tbase.back() + teamT[i]);
-    }
-    qjob.push_back((int)jobs.size());
-    for (int x = 0; x < N_XCD; ++x) {
-        for (int i : queue[x]) {
+    // job lists: [global queue, queue of XCD 0, ..., queue of XCD 7]
+    auto job_list = [&](const std::vector<int> &g, const std::vector<std::vector<int>> &q, std::vector<TriJob> &jobs,
+                        std::vector<int> &tbase, std::vector<int> &qjob) {
+        tbase.assign(1, 0);
+        qjob.assign(1, 0);
+        for (int i : g) {
             jobs.push_back({i, teamT[i]});
             tbase.push_back(tbase.back() + teamT[i]);
         }
         qjob.push_back((int)jobs.size());
-    }
-    const size_t ncnt = N_XCD + 1 + (size_t)ntasks;  // ticket counters + one barrier per job
+        for (int x = 0; x < N_XCD; ++x) {
+            for (int i : q[x]) {
+                jobs.push_back({i, teamT[i]});
+                tbase.push_back(tbase.back() + teamT[i]);
+            }
+            qjob.push_back((int)jobs.size());
+        }
+    };
+    std::vector<TriJob> jobs, rjobs;
+    std::vector<int> tbase, qjob, rtbase, rqjob;
+    job_list(gq, queue, jobs, tbase, qjob);
+    job_list({}, rqueue, rjobs, rtbase, rqjob);
+    // counters: tickets + one barrier per job, register teams first
+    const size_t creg = N_XCD + 1 + rjobs.size(), ncnt = creg + N_XCD + 1 + jobs.size();
     if (ncnt > c->cnt_ints) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (c->cnt) HIPCHK(c, hipFree(c->cnt));
@@ -1411,8 +1693,9 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     }
     HostArena ar;
     const size_t o_jobs = ar.put(jobs), o_tb = ar.put(tbase), o_qj = ar.put(qjob), o_ones = ar.put(ones);
+    const size_t o_rjobs = ar.put(rjobs), o_rtb = ar.put(rtbase), o_rqj = ar.put(rqjob);
     const size_t o_mats = ar.put(mats), o_merges = ar.put(merges), o_sch = ar.put(schunks), o_uch = ar.put(uchunks),
-                 o_gouts = ar.put(gouts), o_gterms = ar.put(gterms), o_gitems = ar.put(gitems), o_rk = ar.put(rk),
+                 o_xch = ar.put(xchunks), o_gouts = ar.put(gouts), o_gterms = ar.put(gterms), o_gitems = ar.put(gitems), o_rk = ar.put(rk),
                  o_cp = ar.put(cp);
     size_t o_bt[kBtVariants];
     for (int v = 0; v < kBtVariants; ++v) o_bt[v] = ar.put(bt[v]);
@@ -1427,21 +1710,25 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     const int *d_tb = (const int *)(db + o_tb);
     const int *d_qj = (const int *)(db + o_qj);
     const DcMerge *d_merges = (const DcMerge *)(db + o_merges);
-    const DcChunk *d_sch = (const DcChunk *)(db + o_sch), *d_uch = (const DcChunk *)(db + o_uch);
+    const DcChunk *d_sch = (const DcChunk *)(db + o_sch), *d_uch = (const DcChunk *)(db + o_uch),
+                  *d_xch = (const DcChunk *)(db + o_xch);
     nrg_gemm_out *d_gouts = (nrg_gemm_out *)(db + o_gouts);
     nrg_gemm_term *d_gterms = (nrg_gemm_term *)(db + o_gterms);
     const GemmItem *d_gitems = (const GemmItem *)(db + o_gitems);
 
-    // 1. tridiagonalisation: matrices below TEAM_MIN_N one workgroup each,
-    //    then one persistent launch for the teams
+    // 1. tridiagonalisation: matrices up to REG_ONE_N one workgroup each,
+    //    then one persistent launch for the register-resident teams and one
+    //    for the streaming teams
     {
-        double fl1 = 0, by1 = 0, flt = 0, byt = 0;
+        double fl1 = 0, by1 = 0, flr = 0, byr = 0, flt = 0, byt = 0;
         for (int i = 0; i < ntasks; ++i) {
             const double n = tasks[i].n;
             // algorithmic work: 4/3 n^3 flops; the trailing matrix is read once
             // per column (sum_j 8 (n-j)^2 bytes)
-            (tasks[i].n < TEAM_MIN_N ? fl1 : flt) += 4.0 / 3.0 * n * n * n;
-            (tasks[i].n < TEAM_MIN_N ? by1 : byt) += 8.0 / 3.0 * n * n * n;
+            double &fl = tasks[i].n <= REG_ONE_N ? fl1 : regjob[i] ? flr : flt;
+            double &by = tasks[i].n <= REG_ONE_N ? by1 : regjob[i] ? byr : byt;
+            fl += 4.0 / 3.0 * n * n * n;
+            by += 8.0 / 3.0 * n * n * n;
         }
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, c->stream, c->cnt, (int)ncnt);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->dstatus, 1);
@@ -1453,18 +1740,24 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
             {
                 LaunchTimer lt(K_EIG_TRI1, c->side, fl1, by1);
-                hipLaunchKernelGGL(k_eig_tri_one, dim3(ones.size()), dim3(TRI_NT),
-                                   5 * (size_t)TRI_LDS_N * sizeof(double), c->side, d_mats, (const int *)(db + o_ones));
+                hipLaunchKernelGGL(k_eig_tri_one, dim3(ones.size()), dim3(TRI_NT), 0, c->side, d_mats,
+                                   (const int *)(db + o_ones), c->dstatus);
             }
             HIPCHK(c, hipEventRecord(c->ev_join, c->side));
         }
-        // grid <= co-resident capacity (one workgroup per CU at this size) and
-        // nothing else runs on the stream, so every team assembles; the
+        // grids <= co-resident capacity (one workgroup per CU at these sizes)
+        // and nothing else runs on the stream, so every team assembles; the
         // barrier's bounded wait reports an error otherwise
+        if (!rjobs.empty()) {
+            LaunchTimer lt(K_EIG_TRI_REG, c->stream, flr, byr);
+            hipLaunchKernelGGL(k_eig_tri_reg, dim3(reg_grid), dim3(REG_NT), 0, c->stream, d_mats,
+                               (const TriJob *)(db + o_rjobs), (const int *)(db + o_rtb), (const int *)(db + o_rqj),
+                               c->cnt, c->dstatus);
+        }
         if (!jobs.empty()) {
             LaunchTimer lt(K_EIG_TRI, c->stream, flt, byt);
             hipLaunchKernelGGL(k_eig_tri_coop, dim3(tri_grid), dim3(TRI_NT), TRI_COOP_LDS, c->stream, d_mats, d_jobs,
-                               d_tb, d_qj, c->cnt, c->dstatus);
+                               d_tb, d_qj, c->cnt + creg, c->dstatus);
         }
         if (!ones.empty()) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     }
@@ -1482,12 +1775,19 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         if (nm <= 0) continue;
         {
             LaunchTimer lt(K_DC_DEFLATE, c->stream, 0.0, 0.0);
+            const int inl = lv.x1 == lv.x0;  // every merge of the level is at most DEFL_INLINE_K
             if (lv.maxK <= DEFL_LDS_K)
                 hipLaunchKernelGGL(k_dc_deflate<true>, dim3(nm), dim3(DEFL_NT), 32 * (size_t)lv.maxK, c->stream,
-                                   d_merges + lv.m0, d_gouts, d_gterms);
+                                   d_merges + lv.m0, d_gouts, d_gterms, inl);
             else
                 hipLaunchKernelGGL(k_dc_deflate<false>, dim3(nm), dim3(DEFL_NT), 0, c->stream, d_merges + lv.m0,
-                                   d_gouts, d_gterms);
+                                   d_gouts, d_gterms, inl);
+        }
+        HIPCHK(c, hipGetLastError());
+        if (lv.x1 > lv.x0) {
+            LaunchTimer lt(K_DC_DEFLATE, c->stream, 0.0, 0.0);
+            hipLaunchKernelGGL(k_dc_gather, dim3(lv.x1 - lv.x0), dim3(GATHER_W), 0, c->stream, d_merges,
+                               d_xch + lv.x0);
         }
         HIPCHK(c, hipGetLastError());
         {

@@ -31,6 +31,7 @@ struct nrg_ctx {
     int *cnt = nullptr;
     size_t cnt_ints = 0;
     int tri_capacity = 0;
+    int reg_capacity = 0;
     // pinned host staging: descriptors, transfers (read / written by kernels
     // directly), the eigensolver status
     void *hdesc = nullptr;
@@ -59,6 +60,7 @@ enum KernelId {
     K_ASSEMBLE,
     K_EIG_TRI,
     K_EIG_TRI1,
+    K_EIG_TRI_REG,
     K_DC_LEAF,
     K_DC_DEFLATE,
     K_DC_SECULAR,

@@ -47,7 +47,10 @@ def _check_eig(A, W, U, tol=1e-12):
 
 def test_diag_batch_random_sizes(ctx):
     rng = np.random.default_rng(1)
-    sizes = [1, 2, 3, 4, 7, 16, 33, 64, 65, 129, 256, 300]
+    # boundaries of the tridiagonalisation variants: one workgroup up to 128,
+    # register teams of 4, 6 and 9 column registers up to 256 / 384 / 576,
+    # streaming teams above
+    sizes = [1, 2, 3, 4, 7, 16, 33, 64, 65, 127, 128, 129, 256, 257, 300, 384, 385, 511, 576, 577]
     mats = [_sym(rng, n) for n in sizes]
     out = ctx.diag_batch(mats)
     for A, (W, U) in zip(mats, out):
@@ -75,6 +78,10 @@ def test_diag_batch_degenerate_and_structured(ctx):
     d = np.sort(rng.uniform(0, 50, 80))
     h = np.diag(d) + 0.3 * _sym(rng, 80)
     mats.append(h)
+    # multiplets in a register-team size: deflation at every merge level
+    Q, _ = np.linalg.qr(rng.standard_normal((400, 400)))
+    lam = np.repeat(np.round(rng.uniform(0, 3, 100), 2), 4)
+    mats.append((Q * lam) @ Q.T)
     out = ctx.diag_batch(mats)
     for A, (W, U) in zip(mats, out):
         _check_eig(A, W, U)
