@@ -27,10 +27,13 @@ sys.path.insert(0, PKG)
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and MFMA)
 PEAK_HBM_GBS = 8000.0
 # What bounds each kernel, and which of its counted quantities is algorithmic:
-# the tridiagonalisation streams the trailing matrix once per column
-# (sum_j 8 (d-j)^2 bytes per block), binning / assembly / transfers are byte
+# the streaming tridiagonalisation (blocks above 576) reads the trailing
+# matrix once per column (sum_j 8 (d-j)^2 bytes per block); the register-
+# resident one (blocks up to 576) keeps its rows on chip, so its ceiling is
+# the FP64 rate of its 4/3 d^3 flops; binning / assembly / transfers are byte
 # streams, the contractions and back-transformations are FP64 arithmetic.
-KERNEL_BOUND = {"k_eig_tri_coop": "hbm", "k_eig_tri_one": "hbm", "k_spec_bin": "hbm", "k_assemble": "hbm", "k_gather_scatter": "hbm",
+KERNEL_BOUND = {"k_eig_tri_coop": "hbm", "k_eig_tri_one": "mfma", "k_eig_tri_reg": "mfma", "k_spec_bin": "hbm",
+                "k_assemble": "hbm", "k_gather_scatter": "hbm",
                 "k_zero": "hbm", "k_eig_copy": "hbm", "k_gemm": "mfma", "k_gemm_dc": "mfma", "k_gemm_bt": "mfma",
                 "k_eig_bt": "mfma"}
 

@@ -158,6 +158,7 @@ constexpr int TEAM_GLOBAL_N = 1024;          // larger matrices: teams across XC
 constexpr int TEAM_GLOBAL_MAX = 128;
 constexpr int N_XCD = 8;                     // workgroup b runs on XCD b mod 8
 constexpr long long TEAM_SPIN_LIMIT = 1LL << 22;  // polls (~1 us each): a few seconds
+constexpr int BAR_STRIDE = 32;                    // ints between team barrier counters (128 B)
 
 struct TriJob {
     int mat, T;
@@ -623,9 +624,9 @@ __global__ __launch_bounds__(REG_NT) void k_eig_tri_reg(const EigMat *mats, cons
         const TriJob jb = jobs[lo];
         const EigMat mt = mats[jb.mat];
         const int rank = t0 + t - tbase[lo];
-        if (mt.n <= 256) tri_reg<4, 8, REG_NT / 64, true>(mt, rank, jb.T, bars + lo, status, &s_flag, nullptr);
-        else if (mt.n <= 384) tri_reg<6, 4, REG_NT / 64, true>(mt, rank, jb.T, bars + lo, status, &s_flag, nullptr);
-        else tri_reg<9, 3, REG_NT / 64, true>(mt, rank, jb.T, bars + lo, status, &s_flag, nullptr);
+        if (mt.n <= 256) tri_reg<4, 8, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, nullptr);
+        else if (mt.n <= 384) tri_reg<6, 4, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, nullptr);
+        else tri_reg<9, 3, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, nullptr);
         __syncthreads();
     }
 }
@@ -635,7 +636,7 @@ __global__ __launch_bounds__(TRI_NT) void k_eig_tri_coop(const EigMat *mats, con
     __shared__ double red[TRI_NW];
     __shared__ int s_ticket, s_flag;
     const int x = blockIdx.x % N_XCD;
-    int *bars = counters + N_XCD + 1;     // counters[9 + job]: team barrier
+    int *bars = counters + N_XCD + 1;     // counters[9 + BAR_STRIDE job]: team barrier
     for (int q = 0; q < 2; ++q) {
     // queue 0: global (jobs qjob[0]..qjob[1]); queue 1: this XCD's
     int *ticket = q == 0 ? counters + N_XCD : counters + x;
@@ -656,8 +657,8 @@ __global__ __launch_bounds__(TRI_NT) void k_eig_tri_coop(const EigMat *mats, con
         const TriJob jb = jobs[lo];
         const EigMat mt = mats[jb.mat];
         const int rank = t0 + t - tbase[lo];
-        if (mt.n <= TEAM_LDS_N) tri_team<true>(mt, rank, jb.T, bars + lo, status, red, &s_flag);
-        else tri_team<false>(mt, rank, jb.T, bars + lo, status, red, &s_flag);
+        if (mt.n <= TEAM_LDS_N) tri_team<true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, red, &s_flag);
+        else tri_team<false>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, red, &s_flag);
         __syncthreads();
     }
     }
@@ -1449,9 +1450,11 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     std::vector<DcChunk> schunks, uchunks, xchunks;
     std::vector<nrg_gemm_out> gouts;
     std::vector<nrg_gemm_term> gterms;
-    std::vector<GemmItem> gitems;
+    std::vector<GemmItem> gitems, gred;  // tiles (or slices), reduce items of split tiles
+    std::vector<int> gks;                 // k-steps bound of each merge output
+    int gslots = 0;                       // partial tiles needed by the largest launch
     struct Level {
-        int m0, m1, s0, s1, u0, u1, g0, g1, x0, x1, maxK;
+        int m0, m1, s0, s1, u0, u1, g0, g1, x0, x1, r0, r1, maxK;
         double flops;
     };
     std::vector<Level> levels(maxlev + 1);
@@ -1480,6 +1483,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                 o.term_count = 1;
                 o.beta = 0.0;
                 gouts.push_back(o);
+                gks.push_back((K + GK - 1) / GK);
                 nrg_gemm_term t{};
                 t.A = ws + 2 * (size_t)n * n + (size_t)a * n + a;
                 t.B = ws + (size_t)n * n + (size_t)a * n + a;
@@ -1492,7 +1496,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                 for (int j0 = 0; j0 < K; j0 += SEC_NT) schunks.push_back({mi, j0});
                 for (int j0 = 0; j0 < K; j0 += 4) uchunks.push_back({mi, j0});
                 for (int tm = 0; tm * GT < K; ++tm)
-                    for (int tn = 0; tn * GT < K; ++tn) gitems.push_back({g, tm, tn, 0});
+                    for (int tn = 0; tn * GT < K; ++tn) gitems.push_back({g, tm, tn, 0, 0, 0});
                 lv.maxK = std::max(lv.maxK, K);
                 lv.flops += 2.0 * K * K * K;
             }
@@ -1500,6 +1504,9 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         lv.m1 = (int)merges.size();
         lv.s1 = (int)schunks.size();
         lv.u1 = (int)uchunks.size();
+        lv.r0 = (int)gred.size();
+        gslots = std::max(gslots, gemm_split(gitems, lv.g0, gks, gred));
+        lv.r1 = (int)gred.size();
         lv.g1 = (int)gitems.size();
         // a level with a merge above DEFL_INLINE_K gathers all its merges in
         // column chunks; otherwise each deflation workgroup gathers its own
@@ -1534,10 +1541,12 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     std::vector<WyItem> wyitems;
     std::vector<nrg_gemm_out> wyouts;
     std::vector<nrg_gemm_term> wyterms;
-    std::vector<GemmItem> wygi;
+    std::vector<GemmItem> wygi, wyred;
+    std::vector<int> wyks;
     struct WyLevel {
         int w0, w1;
         int g[4][2];  // gemm item ranges of G, M, W, Z
+        int r[4][2];  // reduce item ranges
         double flops;
     };
     std::vector<WyLevel> wylev(wy_blocks);
@@ -1582,10 +1591,14 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                 const int g = (int)wyouts.size();
                 wyouts.push_back(o);
                 wyterms.push_back(t);
+                wyks.push_back((t.k + GK - 1) / GK);
                 for (int tm = 0; tm * GT < o.m; ++tm)
-                    for (int tn = 0; tn * GT < o.n; ++tn) wygi.push_back({g, tm, tn, 0});
+                    for (int tn = 0; tn * GT < o.n; ++tn) wygi.push_back({g, tm, tn, 0, 0, 0});
                 if (stage >= 2) L.flops += 2.0 * n * (double)nb * np;
             }
+            L.r[stage][0] = (int)wyred.size();
+            gslots = std::max(gslots, gemm_split(wygi, L.g[stage][0], wyks, wyred));
+            L.r[stage][1] = (int)wyred.size();
             L.g[stage][1] = (int)wygi.size();
         }
         wylev[q] = L;
@@ -1682,8 +1695,11 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     std::vector<int> tbase, qjob, rtbase, rqjob;
     job_list(gq, queue, jobs, tbase, qjob);
     job_list({}, rqueue, rjobs, rtbase, rqjob);
-    // counters: tickets + one barrier per job, register teams first
-    const size_t creg = N_XCD + 1 + rjobs.size(), ncnt = creg + N_XCD + 1 + jobs.size();
+    // counters: tickets + one barrier per job, register teams first; barrier
+    // counters a cache line apart (teams polling neighbouring counters would
+    // contend for one line)
+    const size_t creg = N_XCD + 1 + (size_t)BAR_STRIDE * rjobs.size(),
+                 ncnt = creg + N_XCD + 1 + (size_t)BAR_STRIDE * jobs.size();
     if (ncnt > c->cnt_ints) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (c->cnt) HIPCHK(c, hipFree(c->cnt));
@@ -1695,12 +1711,16 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     const size_t o_jobs = ar.put(jobs), o_tb = ar.put(tbase), o_qj = ar.put(qjob), o_ones = ar.put(ones);
     const size_t o_rjobs = ar.put(rjobs), o_rtb = ar.put(rtbase), o_rqj = ar.put(rqjob);
     const size_t o_mats = ar.put(mats), o_merges = ar.put(merges), o_sch = ar.put(schunks), o_uch = ar.put(uchunks),
-                 o_xch = ar.put(xchunks), o_gouts = ar.put(gouts), o_gterms = ar.put(gterms), o_gitems = ar.put(gitems), o_rk = ar.put(rk),
+                 o_xch = ar.put(xchunks), o_gouts = ar.put(gouts), o_gterms = ar.put(gterms), o_gitems = ar.put(gitems), o_gred = ar.put(gred), o_rk = ar.put(rk),
                  o_cp = ar.put(cp);
     size_t o_bt[kBtVariants];
     for (int v = 0; v < kBtVariants; ++v) o_bt[v] = ar.put(bt[v]);
     const size_t o_wyrows = ar.put(wyrows), o_wyitems = ar.put(wyitems), o_wyouts = ar.put(wyouts),
-                 o_wyterms = ar.put(wyterms), o_wygi = ar.put(wygi);
+                 o_wyterms = ar.put(wyterms), o_wygi = ar.put(wygi), o_wyred = ar.put(wyred);
+    if (gslots) {
+        nrg_status sr = gemm_reserve(c, gslots);
+        if (sr) return sr;
+    }
     void *d;
     nrg_status s = stage(c, ar.buf.data(), ar.buf.size(), &d);
     if (s) return s;
@@ -1714,7 +1734,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                   *d_xch = (const DcChunk *)(db + o_xch);
     nrg_gemm_out *d_gouts = (nrg_gemm_out *)(db + o_gouts);
     nrg_gemm_term *d_gterms = (nrg_gemm_term *)(db + o_gterms);
-    const GemmItem *d_gitems = (const GemmItem *)(db + o_gitems);
+    const GemmItem *d_gitems = (const GemmItem *)(db + o_gitems), *d_gred = (const GemmItem *)(db + o_gred);
 
     // 1. tridiagonalisation: matrices up to REG_ONE_N one workgroup each,
     //    then one persistent launch for the register-resident teams and one
@@ -1803,7 +1823,8 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             hipLaunchKernelGGL(k_dc_unorm, dim3(lv.u1 - lv.u0), dim3(256), 0, c->stream, d_merges, d_uch + lv.u0);
         }
         HIPCHK(c, hipGetLastError());
-        s = gemm_launch_dev(c, K_GEMM_DC, d_gouts, d_gterms, d_gitems + lv.g0, lv.g1 - lv.g0, lv.flops, 0.0);
+        s = gemm_launch_dev(c, K_GEMM_DC, d_gouts, d_gterms, d_gitems + lv.g0, lv.g1 - lv.g0, d_gred + lv.r0,
+                            lv.r1 - lv.r0, lv.flops, 0.0);
         if (s) return s;
     }
     // 3. sort, back-transformation, copy into A
@@ -1839,7 +1860,12 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         const WyItem *d_wyitems = (const WyItem *)(db + o_wyitems);
         const nrg_gemm_out *d_wyouts = (const nrg_gemm_out *)(db + o_wyouts);
         const nrg_gemm_term *d_wyterms = (const nrg_gemm_term *)(db + o_wyterms);
-        const GemmItem *d_wygi = (const GemmItem *)(db + o_wygi);
+        const GemmItem *d_wygi = (const GemmItem *)(db + o_wygi), *d_wyred = (const GemmItem *)(db + o_wyred);
+        auto wy_gemm = [&](const WyLevel &L, int stage, double fl) {
+            return gemm_launch_dev(c, K_GEMM_BT, d_wyouts, d_wyterms, d_wygi + L.g[stage][0],
+                                   L.g[stage][1] - L.g[stage][0], d_wyred + L.r[stage][0],
+                                   L.r[stage][1] - L.r[stage][0], fl, 0.0);
+        };
         {
             LaunchTimer lt(K_EIG_BT, c->stream, 0.0, 0.0);
             hipLaunchKernelGGL(k_wy_extract, dim3(wyrows.size()), dim3(256), 0, c->stream, d_mats, d_wyrows);
@@ -1847,20 +1873,18 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         HIPCHK(c, hipGetLastError());
         for (int q = 0; q < wy_blocks; ++q) {
             const WyLevel &L = wylev[q];
-            s = gemm_launch_dev(c, K_GEMM_BT, d_wyouts, d_wyterms, d_wygi + L.g[0][0], L.g[0][1] - L.g[0][0], 0.0, 0.0);
+            s = wy_gemm(L, 0, 0.0);
             if (s) return s;
             {
                 LaunchTimer lt(K_EIG_BT, c->stream, 0.0, 0.0);
                 hipLaunchKernelGGL(k_wy_tfactor, dim3(L.w1 - L.w0), dim3(WY_NB), 0, c->stream, d_wyitems + L.w0);
             }
             HIPCHK(c, hipGetLastError());
-            s = gemm_launch_dev(c, K_GEMM_BT, d_wyouts, d_wyterms, d_wygi + L.g[1][0], L.g[1][1] - L.g[1][0], 0.0, 0.0);
+            s = wy_gemm(L, 1, 0.0);
             if (s) return s;
-            s = gemm_launch_dev(c, K_GEMM_BT, d_wyouts, d_wyterms, d_wygi + L.g[2][0], L.g[2][1] - L.g[2][0],
-                                0.5 * L.flops, 0.0);
+            s = wy_gemm(L, 2, 0.5 * L.flops);
             if (s) return s;
-            s = gemm_launch_dev(c, K_GEMM_BT, d_wyouts, d_wyterms, d_wygi + L.g[3][0], L.g[3][1] - L.g[3][0],
-                                0.5 * L.flops, 0.0);
+            s = wy_gemm(L, 3, 0.5 * L.flops);
             if (s) return s;
         }
         {

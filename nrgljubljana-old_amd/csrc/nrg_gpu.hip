@@ -159,9 +159,11 @@ __global__ __launch_bounds__(256) void k_assemble(const nrg_asm_block *blocks, c
 
 // ---------------------------------------------------------------------------
 // grouped GEMM (FP64, LDS tiled, 64x64 tile, 4x4 per thread)
+// A workgroup walks the (term, k) sequence of its tile, or of its slice of
+// it, in steps of GK.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_gemm(const nrg_gemm_out *outs, const nrg_gemm_term *terms,
-                                              const GemmItem *items) {
+                                              const GemmItem *items, double *part) {
     __shared__ double As[GK][GT + 1];
     __shared__ double Bs[GK][GT + 1];
     const GemmItem it = items[blockIdx.x];
@@ -170,14 +172,29 @@ __global__ __launch_bounds__(256) void k_gemm(const nrg_gemm_out *outs, const nr
     const int tr = tid / 16, tc = tid % 16;  // 16x16 threads, 4x4 each
     const int row0 = it.tm * GT, col0 = it.tn * GT;
     if (row0 >= o.m || col0 >= o.n) return;  // m may have been lowered on the device
+    // k-steps [g0, g1) of the output's (term, k) sequence belong to this item
+    int g0 = 0, g1 = INT_MAX;
+    if (it.ns > 1) {
+        int S = 0;  // k may have been lowered on the device
+        for (int t = 0; t < o.term_count; ++t) S += (terms[o.term_begin + t].k + GK - 1) / GK;
+        g0 = (int)((long long)S * it.s / it.ns);
+        g1 = (int)((long long)S * (it.s + 1) / it.ns);
+    }
     double acc[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-    for (int t = 0; t < o.term_count; ++t) {
+    int g = 0;
+    for (int t = 0; t < o.term_count && g < g1; ++t) {
         const nrg_gemm_term tm = terms[o.term_begin + t];
-        for (int k0 = 0; k0 < tm.k; k0 += GK) {
+        const int st = (tm.k + GK - 1) / GK;
+        if (g + st <= g0) {  // the whole term lies before this slice
+            g += st;
+            continue;
+        }
+        for (int k0 = 0; k0 < tm.k && g < g1; k0 += GK, ++g) {
+            if (g < g0) continue;
             // load A tile: rows row0..+64, k k0..+16 (1024 elements, 4 per thread)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -231,6 +248,14 @@ __global__ __launch_bounds__(256) void k_gemm(const nrg_gemm_out *outs, const nr
             __syncthreads();
         }
     }
+    if (it.ns > 1) {
+        double *pt = part + (size_t)(it.part + it.s) * GT * GT;
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) pt[(tr + 16 * x) * GT + tc + 16 * y] = acc[x][y];
+        return;
+    }
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
         const int gr = row0 + tr + 16 * x;
@@ -243,6 +268,63 @@ __global__ __launch_bounds__(256) void k_gemm(const nrg_gemm_out *outs, const nr
             *cp = (o.beta == 0.0 ? 0.0 : o.beta * *cp) + acc[x][y];
         }
     }
+}
+
+// C tile = beta C + sum of the slice partials, in slice order
+__global__ __launch_bounds__(256) void k_gemm_reduce(const nrg_gemm_out *outs, const GemmItem *items,
+                                                     const double *part) {
+    const GemmItem it = items[blockIdx.x];
+    const nrg_gemm_out o = outs[it.out];
+    const int row0 = it.tm * GT, col0 = it.tn * GT;
+    if (row0 >= o.m || col0 >= o.n) return;
+    const double *pt = part + (size_t)it.part * GT * GT;
+    for (int e = threadIdx.x; e < GT * GT; e += 256) {
+        const int gr = row0 + e / GT, gc = col0 + e % GT;
+        if (gr >= o.m || gc >= o.n) continue;
+        double v = 0.0;
+        for (int s = 0; s < it.ns; ++s) v += pt[(size_t)s * GT * GT + e];
+        double *cp = o.C + (size_t)gr * o.ldc + gc;
+        *cp = (o.beta == 0.0 ? 0.0 : o.beta * *cp) + v;
+    }
+}
+
+std::vector<int> gemm_ksteps(const nrg_gemm_out *outs, int nouts, const nrg_gemm_term *terms) {
+    std::vector<int> ks(nouts, 0);
+    for (int o = 0; o < nouts; ++o)
+        for (int t = outs[o].term_begin; t < outs[o].term_begin + outs[o].term_count; ++t)
+            ks[o] += (terms[t].k + GK - 1) / GK;
+    return ks;
+}
+
+int gemm_split(std::vector<GemmItem> &items, size_t first, const std::vector<int> &ksteps,
+               std::vector<GemmItem> &reduce) {
+    const size_t ntiles = items.size() - first;
+    if (ntiles == 0 || ntiles >= (size_t)GEMM_SPLIT_MIN) return 0;
+    const int want = (int)((GEMM_SPLIT_TARGET + ntiles - 1) / ntiles);
+    const std::vector<GemmItem> tiles(items.begin() + first, items.end());
+    items.resize(first);
+    int slots = 0;
+    for (const GemmItem &t : tiles) {
+        const int ns = std::max(1, std::min(want, ksteps[t.out] / GEMM_SLICE_MIN));
+        if (ns <= 1) {
+            items.push_back(t);
+            continue;
+        }
+        for (int s = 0; s < ns; ++s) items.push_back({t.out, t.tm, t.tn, s, ns, slots});
+        reduce.push_back({t.out, t.tm, t.tn, 0, ns, slots});
+        slots += ns;
+    }
+    return slots;
+}
+
+nrg_status gemm_reserve(nrg_ctx *c, int slots) {
+    if ((size_t)slots <= c->gpart_tiles) return NRG_OK;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->gpart) HIPCHK(c, hipFree(c->gpart));
+    c->gpart = nullptr;
+    c->gpart_tiles = std::max<size_t>((size_t)slots * 3 / 2, 1024);
+    HIPCHK(c, hipMalloc(&c->gpart, c->gpart_tiles * GT * GT * sizeof(double)));
+    return NRG_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -313,10 +395,13 @@ __global__ __launch_bounds__(256) void k_spec_bin(const nrg_spec_params P, const
 }
 
 nrg_status gemm_launch_dev(nrg_ctx *c, int kid, const nrg_gemm_out *d_outs, const nrg_gemm_term *d_terms,
-                           const GemmItem *d_items, int nitems, double flops, double bytes) {
+                           const GemmItem *d_items, int nitems, const GemmItem *d_reduce, int nreduce,
+                           double flops, double bytes) {
     if (nitems <= 0) return NRG_OK;
     LaunchTimer lt(kid, c->stream, flops, bytes);
-    hipLaunchKernelGGL(k_gemm, dim3(nitems), dim3(256), 0, c->stream, d_outs, d_terms, d_items);
+    hipLaunchKernelGGL(k_gemm, dim3(nitems), dim3(256), 0, c->stream, d_outs, d_terms, d_items, c->gpart);
+    if (nreduce > 0)
+        hipLaunchKernelGGL(k_gemm_reduce, dim3(nreduce), dim3(256), 0, c->stream, d_outs, d_reduce, c->gpart);
     HIPCHK(c, hipGetLastError());
     return NRG_OK;
 }
@@ -354,6 +439,7 @@ void nrg_ctx_destroy(nrg_ctx *c) {
     if (c->desc) (void)hipFree(c->desc);
     if (c->ews) (void)hipFree(c->ews);
     if (c->cnt) (void)hipFree(c->cnt);
+    if (c->gpart) (void)hipFree(c->gpart);
     if (c->dstatus) (void)hipFree(c->dstatus);
     if (c->hdesc) (void)hipHostFree(c->hdesc);
     if (c->hstat) (void)hipHostFree(c->hstat);
@@ -542,15 +628,22 @@ nrg_status nrg_gemm_grouped(nrg_ctx *c, const nrg_gemm_out *outs, int nouts, con
             if (T.k < 0 || T.lda < needA || T.ldb < needB) return fail(c, NRG_ERR_ARG, "gemm: bad term");
         }
         for (int tm = 0; tm * GT < O.m; ++tm)
-            for (int tn = 0; tn * GT < O.n; ++tn) items.push_back({o, tm, tn, 0});
+            for (int tn = 0; tn * GT < O.n; ++tn) items.push_back({o, tm, tn, 0, 0, 0});
     }
     if (items.empty()) return NRG_OK;
+    std::vector<GemmItem> reduce;
+    const int slots = gemm_split(items, 0, gemm_ksteps(outs, nouts, terms), reduce);
+    if (slots) {
+        nrg_status s = gemm_reserve(c, slots);
+        if (s) return s;
+    }
     const size_t ob = sizeof(nrg_gemm_out) * nouts, tb = sizeof(nrg_gemm_term) * nterms,
-                 ib = sizeof(GemmItem) * items.size();
-    std::vector<char> buf(ob + tb + ib);
+                 ib = sizeof(GemmItem) * items.size(), rb = sizeof(GemmItem) * reduce.size();
+    std::vector<char> buf(ob + tb + ib + rb);
     memcpy(buf.data(), outs, ob);
     if (tb) memcpy(buf.data() + ob, terms, tb);
     memcpy(buf.data() + ob + tb, items.data(), ib);
+    if (rb) memcpy(buf.data() + ob + tb + ib, reduce.data(), rb);
     void *d;
     nrg_status s = stage(c, buf.data(), buf.size(), &d);
     if (s) return s;
@@ -562,11 +655,9 @@ nrg_status nrg_gemm_grouped(nrg_ctx *c, const nrg_gemm_out *outs, int nouts, con
             gbytes += 8.0 * (m * k + k * n);
         }
     for (int o = 0; o < nouts; ++o) gbytes += 8.0 * outs[o].m * outs[o].n * (outs[o].beta != 0.0 ? 2 : 1);
-    LaunchTimer lt(K_GEMM, c->stream, gflops, gbytes);
-    hipLaunchKernelGGL(k_gemm, dim3(items.size()), dim3(256), 0, c->stream, (const nrg_gemm_out *)d,
-                       (const nrg_gemm_term *)((char *)d + ob), (const GemmItem *)((char *)d + ob + tb));
-    HIPCHK(c, hipGetLastError());
-    return NRG_OK;
+    return gemm_launch_dev(c, K_GEMM, (const nrg_gemm_out *)d, (const nrg_gemm_term *)((char *)d + ob),
+                           (const GemmItem *)((char *)d + ob + tb), (int)items.size(),
+                           (const GemmItem *)((char *)d + ob + tb + ib), (int)reduce.size(), gflops, gbytes);
 }
 
 nrg_status nrg_spec_bin(nrg_ctx *c, const nrg_spec_params *prm, const nrg_spec_pair *pairs, int npairs,

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -32,6 +33,9 @@ struct nrg_ctx {
     size_t cnt_ints = 0;
     int tri_capacity = 0;
     int reg_capacity = 0;
+    // grouped GEMM: partial tiles of split outputs
+    double *gpart = nullptr;
+    size_t gpart_tiles = 0;
     // pinned host staging: descriptors, transfers (read / written by kernels
     // directly), the eigensolver status
     void *hdesc = nullptr;
@@ -148,14 +152,31 @@ __device__ inline double block_max(double v, double *red) {
 }
 
 // ---------------------------------------------------------------------------
-// grouped GEMM work items (one 64x64 output tile each)
+// grouped GEMM work items: one 64x64 output tile, or slice s of ns of its
+// (term, k) sequence.  Slices write partial tiles (slot part + s of the
+// context's partial buffer); one reduce item per split tile then adds them
+// in slice order into C, so results do not depend on scheduling.
 // ---------------------------------------------------------------------------
 constexpr int GT = 64, GK = 16;
 struct GemmItem {
-    int out, tm, tn, pad;
+    int out, tm, tn;
+    int s, ns, part;  // ns <= 1: the tile is written directly
 };
+// Launch with few tiles: tiles are split until about GEMM_SPLIT_TARGET work
+// items exist, each slice keeping at least GEMM_SLICE_MIN k-steps.
+constexpr int GEMM_SPLIT_MIN = 384, GEMM_SPLIT_TARGET = 1024, GEMM_SLICE_MIN = 8;
+// Splits the tile items [first, end) of one launch when there are few of
+// them; ksteps(o) bounds the k-steps of output o.  Appends one reduce item
+// per split tile; returns the partial slots used (numbered from 0).
+int gemm_split(std::vector<GemmItem> &items, size_t first, const std::vector<int> &ksteps,
+               std::vector<GemmItem> &reduce);
+// k-steps of every output (sum over its terms of ceil(k / GK))
+std::vector<int> gemm_ksteps(const nrg_gemm_out *outs, int nouts, const nrg_gemm_term *terms);
+// Makes room for `slots` partial tiles (synchronises the stream if it grows).
+nrg_status gemm_reserve(nrg_ctx *c, int slots);
 // Launches the grouped GEMM on descriptors that already live in device
 // memory (outputs may have m / k patched by earlier kernels on the stream:
-// tiles past the patched m exit).
+// tiles past the patched m exit), then the reduction of split tiles.
 nrg_status gemm_launch_dev(nrg_ctx *c, int kid, const nrg_gemm_out *d_outs, const nrg_gemm_term *d_terms,
-                           const GemmItem *d_items, int nitems, double flops, double bytes);
+                           const GemmItem *d_items, int nitems, const GemmItem *d_reduce, int nreduce,
+                           double flops, double bytes);

@@ -175,6 +175,51 @@ def test_gemm_grouped_all_transposes_and_scales(ctx):
         np.testing.assert_array_equal(got[:, n:], Cinit[:, n:])  # outside the view untouched
 
 
+def test_gemm_grouped_split_long_k(ctx):
+    # few output tiles with long k: the launch splits every tile's (term, k)
+    # sequence into slices whose partial tiles are reduced in slice order;
+    # a k = 0 term between others must be skipped
+    capi = _capi()
+    rng = np.random.default_rng(11)
+    outs, terms, refs, handles = [], [], [], []
+    for case, (m, n) in enumerate([(70, 130), (64, 64), (1, 200), (129, 3)]):
+        ldc = n + case
+        Cinit = rng.standard_normal((m, ldc))
+        beta = [0.0, 1.0, -0.5, 1.0][case]
+        ref = beta * Cinit[:, :n] if beta != 0.0 else np.zeros((m, n))
+        o = capi.GemmOut()
+        o.C = ctx.put(Cinit)
+        o.m, o.n, o.ldc, o.beta = m, n, ldc, beta
+        o.term_begin = len(terms)
+        ks = [700, 0, 1500, 33]
+        for t_, k in enumerate(ks):
+            tA, tB = bool(t_ & 1), bool(case & 1)
+            A = rng.standard_normal((max(k, 1), m) if tA else (m, max(k, 1)))
+            B = rng.standard_normal((n, max(k, 1)) if tB else (max(k, 1), n))
+            opA = A[:k, :].T if tA else A[:, :k]
+            opB = B[:, :k].T if tB else B[:k, :]
+            s = rng.standard_normal(k) if t_ == 2 else None
+            alpha = float(rng.standard_normal())
+            t = capi.GemmTerm()
+            t.A, t.B = ctx.put(A), ctx.put(B)
+            t.s = ctx.put(s) if s is not None else None
+            t.alpha, t.k = alpha, k
+            t.lda, t.ldb = A.shape[1], B.shape[1]
+            t.transA, t.transB = int(tA), int(tB)
+            terms.append(t)
+            if k:
+                ref = ref + alpha * (opA * (s[None, :] if s is not None else 1.0)) @ opB
+        o.term_count = len(ks)
+        outs.append(o)
+        refs.append((ref, Cinit))
+        handles.append((o.C, m, ldc, n))
+    ctx.gemm_grouped(outs, terms)
+    for (ref, Cinit), (p, m, ldc, n) in zip(refs, handles):
+        got = ctx.get(p, (m, ldc))
+        np.testing.assert_allclose(got[:, :n], ref, rtol=1e-12, atol=1e-11)
+        np.testing.assert_array_equal(got[:, n:], Cinit[:, n:])  # beyond n untouched
+
+
 def test_assemble_batch(ctx):
     capi = _capi()
     rng = np.random.default_rng(5)
