@@ -30,8 +30,8 @@ constexpr int TRI_NT = 1024;
 constexpr int TRI_NW = TRI_NT / 64;
 constexpr int DEFL_NT = 256;
 constexpr int DEFL_LDS_K = 5000;  // 2 x K doubles + 4 x K ints = 32 B x K of dynamic LDS
-constexpr int DEFL_INLINE_K = 512;  // larger merges gather in column chunks of GATHER_W
-constexpr int GATHER_W = 256;
+constexpr int SEG_L = 32;  // gather segments: about one per SEG_L sorted poles
+constexpr int COPY_ROWS = 16;  // deflated rows copied back per workgroup
 // Kind codes of a sorted pole of a merge: bits 0-1 how its row enters the
 // gather (0 deflated by a small z, 1 new pole, 2 rotated into the previous
 // pole); bit 2 set when a later pole rotates it away (deflated); bit 3 marks
@@ -61,7 +61,7 @@ enum VecSlot {
     V_ZH,    // z-hat
     V_TS,    // secular root offsets
     V_ORG,   // int: origin pole of each root
-    V_META,  // int: [a] = K', [a+1] = number of rotations
+    V_META,  // int: [a] = K', [a+1] = number of rotations (K >= 2, so [a+1] is the merge's own)
     V_RHO,   // rho of the merge at [a]
     V_RANK,  // int: final rank of each eigenvalue
     V_KIND,  // int per sorted pole: 0 deflated (small z), 1 pole, 2 rotated into the previous pole
@@ -71,6 +71,8 @@ enum VecSlot {
     V_SZS,
     V_PERM,
     V_POS,
+    V_SEG,   // int: first sorted pole of each gather segment
+    V_NSEG,  // int: [a] = number of gather segments of the merge at a
     V_COUNT
 };
 
@@ -731,14 +733,15 @@ __global__ __launch_bounds__(256) void k_dc_leaf(const EigMat *mats) {
     }
 }
 
-// Column r of the gathered block of one merge [a, a + K) (h = mid - a), in
-// one pass over the sorted poles q: row pos[q] of G (in S1) is eigenvector
-// perm[q] restricted to its own half, with the deflation rotations applied
-// as they come.  A rotation always pairs pole q with the last pole before it
-// that was not deflated by a small z (the carried value), so the carried
-// element stays in a register until the next pole replaces it.  Rows of the
-// deflated poles (pos >= kp) are final and are copied back into Vc.
-__device__ inline void dc_gather_col(double *ws, int n, int a, int h, int K, int kp, int r, const int *perm,
+// Column r of the gathered block of one merge [a, a + K) (h = mid - a) over
+// the sorted poles q0 <= q < q1 of one gather segment: row pos[q] of G (in
+// S1) is eigenvector perm[q] restricted to its own half, with the deflation
+// rotations applied as they come.  A rotation always pairs pole q with the
+// last pole before it that was not deflated by a small z (the carried
+// value), so the carried element stays in a register until the next new
+// pole replaces it; a segment starts at a new pole (or at q = 0), so no
+// carry crosses segments.
+__device__ inline void dc_gather_seg(double *ws, int n, int a, int h, int q0, int q1, int r, const int *perm,
                                      const int *pos, const int *kind, const double *rc, const double *rs) {
     const double *Vc = ws + (size_t)a * n + a;
     double *G = ws + (size_t)n * n + (size_t)a * n + a;
@@ -746,25 +749,25 @@ __device__ inline void dc_gather_col(double *ws, int n, int a, int h, int K, int
     int cr = -1;
     double cv = 0.0;
     constexpr int U = 8;
-    for (int q0 = 0; q0 < K; q0 += U) {
+    for (int qb = q0; qb < q1; qb += U) {
         double v[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int q = q0 + u;
+            const int q = qb + u;
             v[u] = 0.0;
-            if (q < K) {
+            if (q < q1) {
                 const int i = perm[q];
                 if ((i < h) == lo) v[u] = Vc[(size_t)i * n + r];
             }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int q = q0 + u;
-            if (q >= K) break;
+            const int q = qb + u;
+            if (q >= q1) break;
             const int k = kind[q] & KIND_MASK;
             if (k == 0) {
                 G[(size_t)pos[q] * n + r] = v[u];
-            } else if (k == 1) {
+            } else if (k == 1 || cr < 0) {  // (a rotation always has a carried pole)
                 if (cr >= 0) G[(size_t)pos[cr] * n + r] = cv;
                 cr = q;
                 cv = v[u];
@@ -777,16 +780,6 @@ __device__ inline void dc_gather_col(double *ws, int n, int a, int h, int K, int
         }
     }
     if (cr >= 0) G[(size_t)pos[cr] * n + r] = cv;
-    double *Vw = ws + (size_t)a * n + a;
-    for (int p0 = kp; p0 < K; p0 += U) {
-        double v[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (p0 + u < K) v[u] = G[(size_t)(p0 + u) * n + r];
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (p0 + u < K) Vw[(size_t)(p0 + u) * n + r] = v[u];
-    }
 }
 
 // Ascending sort of (key[i], idx[i]), i < K, by the total order (key, idx):
@@ -867,8 +860,8 @@ __device__ inline bool dc_close(double zp, double zq, double dp, double dq, doub
     return fabs(t * c * s) <= tol;
 }
 
-// Sort, z and deflation of one merge [a, b) with boundary mid; merges up to
-// DEFL_INLINE_K also gather here (inline_gather), larger ones in k_dc_gather.
+// Sort, z, deflation and gather segments of one merge [a, b) with boundary
+// mid (the gather itself runs in k_dc_gather).
 // Deflation follows the sequential rule of LAPACK dlaed2: a pole with a
 // small z component is deflated; otherwise, when it is close to the previous
 // undeflated pole, a Givens rotation moves that pole's z into it and the
@@ -878,7 +871,7 @@ __device__ inline bool dc_close(double zp, double zq, double dp, double dq, doub
 // that follow a rotation.
 template <bool IN_LDS>
 __global__ __launch_bounds__(DEFL_NT) void k_dc_deflate(const DcMerge *merges, nrg_gemm_out *outs,
-                                                      nrg_gemm_term *terms, int inline_gather) {
+                                                      nrg_gemm_term *terms) {
     extern __shared__ double lds[];
     __shared__ double red[DEFL_NT / 64];
     __shared__ int sred[DEFL_NT / 64];
@@ -1041,29 +1034,75 @@ __global__ __launch_bounds__(DEFL_NT) void k_dc_deflate(const DcMerge *merges, n
             D[p] = ds[q];  // deflated eigenvalue, final for this merge
         }
     }
-    if (inline_gather) {
-        for (int r = tid; r < K; r += DEFL_NT) dc_gather_col(ws, n, a, h, K, kp, r, perm, pos, kind, rc, rsn);
-    } else if (IN_LDS) {
-        // the column-chunk gather reads the permutation from global memory
+    if (IN_LDS) {
+        // the gather reads the permutation from global memory
         int *gperm = islot(ws, n, V_PERM) + a, *gpos = islot(ws, n, V_POS) + a;
         for (int q = tid; q < K; q += DEFL_NT) {
             gperm[q] = perm[q];
             gpos[q] = pos[q];
         }
     }
+    // gather segments: q = 0 and the first new pole (kind 1) of every block
+    // of SEG_L sorted poles (cand is free again)
+    const int nblk = (K + SEG_L - 1) / SEG_L;
+    for (int i = tid; i < nblk; i += DEFL_NT) {
+        int f = -1;
+        for (int q = max(1, i * SEG_L); q < min(K, (i + 1) * SEG_L); ++q)
+            if ((kind[q] & KIND_MASK) == 1) {
+                f = q;
+                break;
+            }
+        cand[i] = f;
+    }
+    __syncthreads();
+    {
+        const int bper = (nblk + DEFL_NT - 1) / DEFL_NT, b0 = min(nblk, tid * bper), b1 = min(nblk, b0 + bper);
+        int cnt = 0;
+        for (int b = b0; b < b1; ++b) cnt += cand[b] >= 0;
+        int nseg;
+        int sb = 1 + dc_block_scan<false>(cnt, sred, nseg);
+        int *seg = islot(ws, n, V_SEG) + a;
+        for (int b = b0; b < b1; ++b)
+            if (cand[b] >= 0) seg[sb++] = cand[b];
+        if (tid == 0) {
+            seg[0] = 0;
+            islot(ws, n, V_NSEG)[a] = nseg + 1;
+        }
+    }
 }
 
-// Gather of columns [c0, c0 + GATHER_W) of one merge, one column per thread
-// (the deflation rotations mix rows only, so columns are independent).
-__global__ __launch_bounds__(GATHER_W) void k_dc_gather(const DcMerge *merges, const DcChunk *chunks) {
-    const DcChunk ck = chunks[blockIdx.x];
-    const DcMerge mg = merges[ck.merge];
+struct GatherItem {
+    int merge, s0, c0, pad;
+};
+
+// Gather: wavefront w of item (merge, s0, c0) takes segment s0 + w, lane l
+// column c0 + l (the deflation rotations mix rows only, so columns are
+// independent; segments are independent by construction).
+__global__ __launch_bounds__(256) void k_dc_gather(const DcMerge *merges, const GatherItem *items) {
+    const GatherItem it = items[blockIdx.x];
+    const DcMerge mg = merges[it.merge];
     const int n = mg.n, a = mg.a, K = mg.b - mg.a;
-    const int r = ck.j0 + threadIdx.x;
-    if (r >= K) return;
     double *ws = mg.ws;
-    dc_gather_col(ws, n, a, mg.mid - a, K, islot(ws, n, V_META)[a], r, islot(ws, n, V_PERM) + a,
-                  islot(ws, n, V_POS) + a, islot(ws, n, V_KIND) + a, vslot(ws, n, V_RC) + a, vslot(ws, n, V_RS) + a);
+    const int nseg = islot(ws, n, V_NSEG)[a];
+    const int sgi = it.s0 + (threadIdx.x >> 6), r = it.c0 + (threadIdx.x & 63);
+    if (sgi >= nseg || r >= K) return;
+    const int *seg = islot(ws, n, V_SEG) + a;
+    const int q0 = max(0, seg[sgi]), q1 = min(K, sgi + 1 < nseg ? seg[sgi + 1] : K);
+    dc_gather_seg(ws, n, a, mg.mid - a, q0, q1, r, islot(ws, n, V_PERM) + a, islot(ws, n, V_POS) + a,
+                  islot(ws, n, V_KIND) + a, vslot(ws, n, V_RC) + a, vslot(ws, n, V_RS) + a);
+}
+
+// Rows of the deflated poles (positions kp..K-1 of G) are final for this
+// merge: copied back into Vc once the whole gather has read Vc.
+__device__ inline void dc_copy_deflated(const DcMerge &mg, int j0) {
+    const int n = mg.n, a = mg.a, K = mg.b - mg.a;
+    double *ws = mg.ws;
+    const int kp = islot(ws, n, V_META)[a];
+    const int p0 = kp + j0, p1 = min(K, p0 + COPY_ROWS);
+    const double *G = ws + (size_t)n * n + (size_t)a * n + a;
+    double *Vc = ws + (size_t)a * n + a;
+    for (int p = p0; p < p1; ++p)
+        for (int c = threadIdx.x; c < K; c += blockDim.x) Vc[(size_t)p * n + c] = G[(size_t)p * n + c];
 }
 
 // Root j of 1/rho + sum_i zz_i^2 / (dd_i - lambda) = 0 (dd ascending,
@@ -1168,7 +1207,14 @@ __device__ inline void secular_root(const double *dd, const double *zz, int K, d
 
 // One wavefront per root: the root, then its row of Delta_ji = (dd_i -
 // dd_org(j)) - tau_j.
-__global__ __launch_bounds__(256) void k_dc_secular(const DcMerge *merges, const DcChunk *chunks) {
+// Workgroups past nsec copy deflated rows back (items of dc_copy_deflated).
+__global__ __launch_bounds__(256) void k_dc_secular(const DcMerge *merges, const DcChunk *chunks, int nsec,
+                                                    const DcChunk *copies) {
+    if ((int)blockIdx.x >= nsec) {
+        const DcChunk cp = copies[blockIdx.x - nsec];
+        dc_copy_deflated(merges[cp.merge], cp.j0);
+        return;
+    }
     const DcChunk ck = chunks[blockIdx.x];
     const DcMerge mg = merges[ck.merge];
     const int n = mg.n, a = mg.a;
@@ -1447,14 +1493,15 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     int maxlev = 0;
     for (int i = 0; i < nbig; ++i) maxlev = std::max(maxlev, dc_levels(tasks[i].n));
     std::vector<DcMerge> merges;
-    std::vector<DcChunk> schunks, uchunks, xchunks;
+    std::vector<DcChunk> schunks, uchunks, copies;  // copies: deflated-row chunks (upper bound)
+    std::vector<GatherItem> gathers;
     std::vector<nrg_gemm_out> gouts;
     std::vector<nrg_gemm_term> gterms;
     std::vector<GemmItem> gitems, gred;  // tiles (or slices), reduce items of split tiles
     std::vector<int> gks;                 // k-steps bound of each merge output
     int gslots = 0;                       // partial tiles needed by the largest launch
     struct Level {
-        int m0, m1, s0, s1, u0, u1, g0, g1, x0, x1, r0, r1, maxK;
+        int m0, m1, s0, s1, u0, u1, g0, g1, x0, x1, c0, c1, r0, r1, maxK;
         double flops;
     };
     std::vector<Level> levels(maxlev + 1);
@@ -1464,7 +1511,8 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         lv.s0 = (int)schunks.size();
         lv.u0 = (int)uchunks.size();
         lv.g0 = (int)gitems.size();
-        lv.x0 = (int)xchunks.size();
+        lv.x0 = (int)gathers.size();
+        lv.c0 = (int)copies.size();
         for (int i = 0; i < nbig; ++i) {
             const int n = tasks[i].n, nlev = dc_levels(n);
             if (L > nlev) continue;
@@ -1495,6 +1543,11 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                 gterms.push_back(t);
                 for (int j0 = 0; j0 < K; j0 += SEC_NT) schunks.push_back({mi, j0});
                 for (int j0 = 0; j0 < K; j0 += 4) uchunks.push_back({mi, j0});
+                // gather: at most ceil(K / SEG_L) + 1 segments, four per workgroup
+                const int nseg_max = (K + SEG_L - 1) / SEG_L + 1;
+                for (int s0 = 0; s0 < nseg_max; s0 += 4)
+                    for (int c0 = 0; c0 < K; c0 += 64) gathers.push_back({mi, s0, c0, 0});
+                for (int j0 = 0; j0 < K; j0 += COPY_ROWS) copies.push_back({mi, j0});
                 for (int tm = 0; tm * GT < K; ++tm)
                     for (int tn = 0; tn * GT < K; ++tn) gitems.push_back({g, tm, tn, 0, 0, 0});
                 lv.maxK = std::max(lv.maxK, K);
@@ -1508,12 +1561,8 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         gslots = std::max(gslots, gemm_split(gitems, lv.g0, gks, gred));
         lv.r1 = (int)gred.size();
         lv.g1 = (int)gitems.size();
-        // a level with a merge above DEFL_INLINE_K gathers all its merges in
-        // column chunks; otherwise each deflation workgroup gathers its own
-        if (lv.maxK > DEFL_INLINE_K)
-            for (int mi = lv.m0; mi < lv.m1; ++mi)
-                for (int j0 = 0; j0 < merges[mi].b - merges[mi].a; j0 += GATHER_W) xchunks.push_back({mi, j0});
-        lv.x1 = (int)xchunks.size();
+        lv.x1 = (int)gathers.size();
+        lv.c1 = (int)copies.size();
         levels[L] = lv;
     }
     // back-transformation items by register-slot variant; rank / copy items
@@ -1711,7 +1760,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     const size_t o_jobs = ar.put(jobs), o_tb = ar.put(tbase), o_qj = ar.put(qjob), o_ones = ar.put(ones);
     const size_t o_rjobs = ar.put(rjobs), o_rtb = ar.put(rtbase), o_rqj = ar.put(rqjob);
     const size_t o_mats = ar.put(mats), o_merges = ar.put(merges), o_sch = ar.put(schunks), o_uch = ar.put(uchunks),
-                 o_xch = ar.put(xchunks), o_gouts = ar.put(gouts), o_gterms = ar.put(gterms), o_gitems = ar.put(gitems), o_gred = ar.put(gred), o_rk = ar.put(rk),
+                 o_xch = ar.put(gathers), o_cpy = ar.put(copies), o_gouts = ar.put(gouts), o_gterms = ar.put(gterms), o_gitems = ar.put(gitems), o_gred = ar.put(gred), o_rk = ar.put(rk),
                  o_cp = ar.put(cp);
     size_t o_bt[kBtVariants];
     for (int v = 0; v < kBtVariants; ++v) o_bt[v] = ar.put(bt[v]);
@@ -1731,7 +1780,8 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     const int *d_qj = (const int *)(db + o_qj);
     const DcMerge *d_merges = (const DcMerge *)(db + o_merges);
     const DcChunk *d_sch = (const DcChunk *)(db + o_sch), *d_uch = (const DcChunk *)(db + o_uch),
-                  *d_xch = (const DcChunk *)(db + o_xch);
+                  *d_cpy = (const DcChunk *)(db + o_cpy);
+    const GatherItem *d_xch = (const GatherItem *)(db + o_xch);
     nrg_gemm_out *d_gouts = (nrg_gemm_out *)(db + o_gouts);
     nrg_gemm_term *d_gterms = (nrg_gemm_term *)(db + o_gterms);
     const GemmItem *d_gitems = (const GemmItem *)(db + o_gitems), *d_gred = (const GemmItem *)(db + o_gred);
@@ -1795,24 +1845,26 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         if (nm <= 0) continue;
         {
             LaunchTimer lt(K_DC_DEFLATE, c->stream, 0.0, 0.0);
-            const int inl = lv.x1 == lv.x0;  // every merge of the level is at most DEFL_INLINE_K
             if (lv.maxK <= DEFL_LDS_K)
                 hipLaunchKernelGGL(k_dc_deflate<true>, dim3(nm), dim3(DEFL_NT), 32 * (size_t)lv.maxK, c->stream,
-                                   d_merges + lv.m0, d_gouts, d_gterms, inl);
+                                   d_merges + lv.m0, d_gouts, d_gterms);
             else
                 hipLaunchKernelGGL(k_dc_deflate<false>, dim3(nm), dim3(DEFL_NT), 0, c->stream, d_merges + lv.m0,
-                                   d_gouts, d_gterms, inl);
-        }
-        HIPCHK(c, hipGetLastError());
-        if (lv.x1 > lv.x0) {
-            LaunchTimer lt(K_DC_DEFLATE, c->stream, 0.0, 0.0);
-            hipLaunchKernelGGL(k_dc_gather, dim3(lv.x1 - lv.x0), dim3(GATHER_W), 0, c->stream, d_merges,
-                               d_xch + lv.x0);
+                                   d_gouts, d_gterms);
         }
         HIPCHK(c, hipGetLastError());
         {
+            LaunchTimer lt(K_DC_DEFLATE, c->stream, 0.0, 0.0);
+            hipLaunchKernelGGL(k_dc_gather, dim3(lv.x1 - lv.x0), dim3(256), 0, c->stream, d_merges, d_xch + lv.x0);
+        }
+        HIPCHK(c, hipGetLastError());
+        {
+            // the secular equations, and the copy of deflated rows once the
+            // gather has finished reading Vc
             LaunchTimer lt(K_DC_SECULAR, c->stream, 0.0, 0.0);
-            hipLaunchKernelGGL(k_dc_secular, dim3(lv.u1 - lv.u0), dim3(256), 0, c->stream, d_merges, d_uch + lv.u0);
+            const int nsec = lv.u1 - lv.u0;
+            hipLaunchKernelGGL(k_dc_secular, dim3(nsec + lv.c1 - lv.c0), dim3(256), 0, c->stream, d_merges,
+                               d_uch + lv.u0, nsec, d_cpy + lv.c0);
         }
         {
             LaunchTimer lt(K_DC_ZHAT, c->stream, 0.0, 0.0);
