@@ -76,14 +76,21 @@ nrg_status nrg_zero(nrg_ctx *ctx, double *dst, size_t count);
 /* Batched transfers of many small strided device vectors (eigenvalues of all
  * subspaces, density-matrix and operator diagonals): item k is `count`
  * doubles at ptr[0], ptr[stride], ...; the host side is all items back to
- * back.  One gather/scatter kernel and one copy per call.  No counterpart in
- * the reference, which keeps these vectors in host memory. */
+ * back.  One gather/scatter kernel per call.  The gather returns once the
+ * values are on the host (it waits for the stream); the scatter is queued
+ * like a launch (the values travel with its staged descriptors, so `host`
+ * may be reused when it returns).  No counterpart in the reference, which
+ * keeps these vectors in host memory. */
 typedef struct {
     double *ptr;
     int64_t count;
     int64_t stride;
 } nrg_strided;
 nrg_status nrg_gather_d2h(nrg_ctx *ctx, const nrg_strided *items, int nitems, double *host);
+/* The same gather into device memory `dst`, queued like a launch: values
+ * that the host needs only later (a trace per iteration) are collected on
+ * the device and read back together. */
+nrg_status nrg_gather_dev(nrg_ctx *ctx, const nrg_strided *items, int nitems, double *dst);
 nrg_status nrg_scatter_h2d(nrg_ctx *ctx, const nrg_strided *items, int nitems, const double *host);
 
 /* ---- assembly ----------------------------------------------------------- */
@@ -225,6 +232,9 @@ typedef struct {
 } nrg_kernel_stat;
 
 void nrg_timing_enable(int on);
+/* Records events only for launches of the named kernel (NULL or "" = all),
+ * so that a timed run carries the events of the kernel under study alone. */
+void nrg_timing_filter(const char *kernel);
 void nrg_timing_reset(void);
 /* Fills up to `max` entries, returns the number of kernels with launches. */
 int nrg_timing_get(nrg_kernel_stat *out, int max);

@@ -1948,7 +1948,8 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     // the status word reaches the host through pinned memory (no copy)
     hipLaunchKernelGGL(k_copy_int, dim3(1), dim3(64), 0, c->stream, c->dstatus, c->hstat);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    s = wait_stream(c);
+    if (s) return s;
     if (*c->hstat) return fail(c, NRG_ERR_HIP, "diag: a tridiagonalisation team did not assemble in time");
     return NRG_OK;
 }

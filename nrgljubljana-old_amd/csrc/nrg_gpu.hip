@@ -12,12 +12,16 @@
 //
 // Every launch is on the context's stream; descriptor arrays are staged in
 // a context-owned device buffer.
+#include <chrono>
+#include <thread>
+
 #include "nrg_internal.h"
 
 const char *kKernelNames[K_COUNT] = {"k_zero",       "k_assemble", "k_eig_tri_coop", "k_eig_tri_one", "k_eig_tri_reg", "k_dc_leaf", "k_dc_deflate",
                                      "k_dc_secular", "k_dc_zhat",  "k_dc_unorm", "k_gemm_dc", "k_gemm_bt", "k_dc_rank",
                                      "k_eig_bt",     "k_eig_copy", "k_gemm",     "k_spec_bin", "k_gather_scatter"};
 bool g_timing = false;
+int g_timing_only = -1;
 nrg_kernel_stat g_stats[K_COUNT];
 std::vector<PendingLaunch> g_pending;
 
@@ -35,23 +39,51 @@ void timing_fold() {
     g_pending.clear();
 }
 
-nrg_status stage(nrg_ctx *c, const void *host, size_t bytes, void **dev) {
-    if (bytes > c->desc_bytes) {
-        if (c->desc) {
-            HIPCHK(c, hipStreamSynchronize(c->stream));
-            HIPCHK(c, hipFree(c->desc));
-            HIPCHK(c, hipHostFree(c->hdesc));
-        }
-        size_t nb = std::max<size_t>(bytes, 1 << 20);
-        HIPCHK(c, hipMalloc(&c->desc, nb));
-        HIPCHK(c, hipHostMalloc(&c->hdesc, nb, hipHostMallocDefault));
-        c->desc_bytes = nb;
+nrg_status wait_event(nrg_ctx *c, hipEvent_t e) {
+    // spin for a short while, then yield between polls (solves running in
+    // other threads of a serving process keep their cores)
+    const auto t0 = std::chrono::steady_clock::now();
+    for (bool spin = true;;) {
+        const hipError_t r = hipEventQuery(e);
+        if (r == hipSuccess) return NRG_OK;
+        if (r != hipErrorNotReady) return fail(c, NRG_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(r));
+        if (spin) spin = std::chrono::steady_clock::now() - t0 < std::chrono::microseconds(200);
+        else std::this_thread::yield();
     }
-    // The staging buffers are reused by the next call: wait for previous users.
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    memcpy(c->hdesc, host, bytes);
-    HIPCHK(c, hipMemcpyAsync(c->desc, c->hdesc, bytes, hipMemcpyHostToDevice, c->stream));
-    *dev = c->desc;
+}
+
+nrg_status wait_stream(nrg_ctx *c) {
+    HIPCHK(c, hipEventRecord(c->ev_wait, c->stream));
+    return wait_event(c, c->ev_wait);
+}
+
+nrg_status stage(nrg_ctx *c, const void *host, size_t bytes, void **dev) {
+    // the current slot is released by everything queued so far
+    if (c->desc_cur >= 0) {
+        HIPCHK(c, hipEventRecord(c->desc_ev[c->desc_cur], c->stream));
+        c->desc_busy[c->desc_cur] = true;
+    }
+    const int k = (c->desc_cur + 1) % nrg_ctx::NSTAGE;
+    c->desc_cur = k;
+    if (c->desc_busy[k]) {
+        nrg_status s = wait_event(c, c->desc_ev[k]);
+        if (s) return s;
+        c->desc_busy[k] = false;
+    }
+    if (bytes > c->desc_bytes[k]) {
+        if (c->desc[k]) {
+            HIPCHK(c, hipFree(c->desc[k]));
+            HIPCHK(c, hipHostFree(c->hdesc[k]));
+            c->desc[k] = c->hdesc[k] = nullptr;
+        }
+        const size_t nb = std::max<size_t>(bytes + bytes / 2, 1 << 20);
+        HIPCHK(c, hipMalloc(&c->desc[k], nb));
+        HIPCHK(c, hipHostMalloc(&c->hdesc[k], nb, hipHostMallocDefault));
+        c->desc_bytes[k] = nb;
+    }
+    memcpy(c->hdesc[k], host, bytes);
+    HIPCHK(c, hipMemcpyAsync(c->desc[k], c->hdesc[k], bytes, hipMemcpyHostToDevice, c->stream));
+    *dev = c->desc[k];
     return NRG_OK;
 }
 
@@ -89,10 +121,12 @@ __global__ __launch_bounds__(256) void k_scatter(const XItem *items, const XChun
     for (long long e = ck.e0 + threadIdx.x; e < ck.e1; e += 256) it.ptr[e * it.stride] = buf[it.off + e];
 }
 
-// Stages the item list and sizes the device / pinned buffers; returns the
-// total element count.
-static nrg_status xfer_prepare(nrg_ctx *c, const nrg_strided *items, int nitems, size_t *total, XItem **d_items,
-                               XChunk **d_chunks, int *nchunks) {
+// Stages the item list (and, for a scatter, the host values after it);
+// returns the total element count.  A gather to the host needs the pinned
+// buffer sized (to_host).
+static nrg_status xfer_prepare(nrg_ctx *c, const nrg_strided *items, int nitems, const double *payload, bool to_host,
+                               size_t *total, XItem **d_items, XChunk **d_chunks, int *nchunks,
+                               const double **d_payload) {
     std::vector<XItem> xi(nitems);
     std::vector<XChunk> xc;
     long long off = 0;
@@ -107,7 +141,7 @@ static nrg_status xfer_prepare(nrg_ctx *c, const nrg_strided *items, int nitems,
     *total = (size_t)off;
     *nchunks = (int)xc.size();
     if (!off) return NRG_OK;
-    if ((size_t)off > c->hpin_doubles) {
+    if (to_host && (size_t)off > c->hpin_doubles) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (c->hpin) HIPCHK(c, hipHostFree(c->hpin));
         c->hpin = nullptr;
@@ -115,15 +149,18 @@ static nrg_status xfer_prepare(nrg_ctx *c, const nrg_strided *items, int nitems,
         HIPCHK(c, hipHostMalloc((void **)&c->hpin, nd * sizeof(double), hipHostMallocDefault));
         c->hpin_doubles = nd;
     }
-    const size_t ib = sizeof(XItem) * xi.size();
-    std::vector<char> buf(ib + sizeof(XChunk) * xc.size());
+    const size_t ib = sizeof(XItem) * xi.size(), cb = sizeof(XChunk) * xc.size();
+    const size_t pb = (ib + cb + 7) & ~(size_t)7, vb = payload ? sizeof(double) * (size_t)off : 0;
+    std::vector<char> buf(pb + vb);
     memcpy(buf.data(), xi.data(), ib);
-    memcpy(buf.data() + ib, xc.data(), sizeof(XChunk) * xc.size());
+    memcpy(buf.data() + ib, xc.data(), cb);
+    if (vb) memcpy(buf.data() + pb, payload, vb);
     void *d;
     nrg_status s = stage(c, buf.data(), buf.size(), &d);
     if (s) return s;
     *d_items = (XItem *)d;
     *d_chunks = (XChunk *)((char *)d + ib);
+    if (d_payload) *d_payload = (const double *)((char *)d + pb);
     return NRG_OK;
 }
 
@@ -420,8 +457,11 @@ nrg_status nrg_ctx_create(nrg_ctx **out, int device) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_wait, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&c->dstatus, sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->hstat, sizeof(int), hipHostMallocDefault);
+    for (int k = 0; k < nrg_ctx::NSTAGE && e == hipSuccess; ++k)
+        e = hipEventCreateWithFlags(&c->desc_ev[k], hipEventDisableTiming);
     if (e != hipSuccess) {
         fprintf(stderr, "nrg_ctx_create: %s\n", hipGetErrorString(e));
         delete c;
@@ -436,17 +476,21 @@ void nrg_ctx_destroy(nrg_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->desc) (void)hipFree(c->desc);
+    for (int k = 0; k < nrg_ctx::NSTAGE; ++k) {
+        if (c->desc[k]) (void)hipFree(c->desc[k]);
+        if (c->hdesc[k]) (void)hipHostFree(c->hdesc[k]);
+        if (c->desc_ev[k]) (void)hipEventDestroy(c->desc_ev[k]);
+    }
     if (c->ews) (void)hipFree(c->ews);
     if (c->cnt) (void)hipFree(c->cnt);
     if (c->gpart) (void)hipFree(c->gpart);
     if (c->dstatus) (void)hipFree(c->dstatus);
-    if (c->hdesc) (void)hipHostFree(c->hdesc);
     if (c->hstat) (void)hipHostFree(c->hstat);
     if (c->hpin) (void)hipHostFree(c->hpin);
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->ev_wait) (void)hipEventDestroy(c->ev_wait);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -455,12 +499,20 @@ void nrg_ctx_destroy(nrg_ctx *c) {
 const char *nrg_last_error(const nrg_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
 nrg_status nrg_sync(nrg_ctx *c) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    nrg_status s = wait_stream(c);
+    if (s) return s;
     if (!g_pending.empty()) timing_fold();
     return NRG_OK;
 }
 
 void nrg_timing_enable(int on) { g_timing = on != 0; }
+
+void nrg_timing_filter(const char *kernel) {
+    g_timing_only = -1;
+    if (!kernel || !*kernel) return;
+    for (int k = 0; k < K_COUNT; ++k)
+        if (!strcmp(kernel, kKernelNames[k])) g_timing_only = k;
+}
 
 void nrg_timing_reset(void) {
     timing_fold();
@@ -498,23 +550,20 @@ nrg_status nrg_dev_free(nrg_ctx *c, void *dptr) {
 nrg_status nrg_h2d(nrg_ctx *c, void *dst, const void *src, size_t bytes) {
     if (!bytes) return NRG_OK;
     HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return NRG_OK;
+    return wait_stream(c);
 }
 
 nrg_status nrg_d2h(nrg_ctx *c, void *dst, const void *src, size_t bytes) {
     if (!bytes) return NRG_OK;
     HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return NRG_OK;
+    return wait_stream(c);
 }
 
 nrg_status nrg_d2h_strided(nrg_ctx *c, double *dst, const double *src, size_t count, size_t stride) {
     if (!count) return NRG_OK;
     HIPCHK(c, hipMemcpy2DAsync(dst, sizeof(double), src, stride * sizeof(double), sizeof(double), count,
                                hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return NRG_OK;
+    return wait_stream(c);
 }
 
 nrg_status nrg_gather_d2h(nrg_ctx *c, const nrg_strided *items, int nitems, double *host) {
@@ -523,7 +572,7 @@ nrg_status nrg_gather_d2h(nrg_ctx *c, const nrg_strided *items, int nitems, doub
     XItem *di = nullptr;
     XChunk *dc = nullptr;
     int nch = 0;
-    nrg_status s = xfer_prepare(c, items, nitems, &tot, &di, &dc, &nch);
+    nrg_status s = xfer_prepare(c, items, nitems, nullptr, true, &tot, &di, &dc, &nch, nullptr);
     if (s || !tot) return s;
     {
         // the kernel writes straight into the pinned host buffer (no copy)
@@ -531,8 +580,26 @@ nrg_status nrg_gather_d2h(nrg_ctx *c, const nrg_strided *items, int nitems, doub
         hipLaunchKernelGGL(k_gather, dim3(nch), dim3(256), 0, c->stream, di, dc, c->hpin);
     }
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    s = wait_stream(c);
+    if (s) return s;
     memcpy(host, c->hpin, tot * sizeof(double));
+    return NRG_OK;
+}
+
+nrg_status nrg_gather_dev(nrg_ctx *c, const nrg_strided *items, int nitems, double *dst) {
+    if (nitems <= 0) return NRG_OK;
+    if (!dst) return fail(c, NRG_ERR_ARG, "gather_dev: no destination");
+    size_t tot = 0;
+    XItem *di = nullptr;
+    XChunk *dc = nullptr;
+    int nch = 0;
+    nrg_status s = xfer_prepare(c, items, nitems, nullptr, false, &tot, &di, &dc, &nch, nullptr);
+    if (s || !tot) return s;
+    {
+        LaunchTimer lt(K_XFER, c->stream, 0.0, 16.0 * tot);
+        hipLaunchKernelGGL(k_gather, dim3(nch), dim3(256), 0, c->stream, di, dc, dst);
+    }
+    HIPCHK(c, hipGetLastError());
     return NRG_OK;
 }
 
@@ -542,17 +609,16 @@ nrg_status nrg_scatter_h2d(nrg_ctx *c, const nrg_strided *items, int nitems, con
     XItem *di = nullptr;
     XChunk *dc = nullptr;
     int nch = 0;
-    nrg_status s = xfer_prepare(c, items, nitems, &tot, &di, &dc, &nch);
+    // the values travel with the staged items (asynchronous: the host
+    // buffer may be reused as soon as the call returns)
+    const double *dv = nullptr;
+    nrg_status s = xfer_prepare(c, items, nitems, host, false, &tot, &di, &dc, &nch, &dv);
     if (s || !tot) return s;
-    memcpy(c->hpin, host, tot * sizeof(double));
     {
-        // the kernel reads straight from the pinned host buffer (no copy)
         LaunchTimer lt(K_XFER, c->stream, 0.0, 16.0 * tot);
-        hipLaunchKernelGGL(k_scatter, dim3(nch), dim3(256), 0, c->stream, di, dc, c->hpin);
+        hipLaunchKernelGGL(k_scatter, dim3(nch), dim3(256), 0, c->stream, di, dc, dv);
     }
     HIPCHK(c, hipGetLastError());
-    // the pinned buffer and the staged items are reused by the next call
-    HIPCHK(c, hipStreamSynchronize(c->stream));
     return NRG_OK;
 }
 

@@ -20,10 +20,18 @@ struct nrg_ctx {
     // side stream (default priority) for work that overlaps the main stream
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_wait = nullptr;  // recorded by wait_stream
     std::string err;
-    // staging for descriptor arrays
-    void *desc = nullptr;
-    size_t desc_bytes = 0;
+    // staging ring for descriptor arrays: slot k (device buffer, pinned host
+    // buffer) is reused once the event recorded when the next slot was taken
+    // has completed, so the host runs up to NSTAGE launches ahead of the GPU
+    static constexpr int NSTAGE = 8;
+    void *desc[NSTAGE] = {};
+    void *hdesc[NSTAGE] = {};
+    size_t desc_bytes[NSTAGE] = {};
+    hipEvent_t desc_ev[NSTAGE] = {};
+    bool desc_busy[NSTAGE] = {};
+    int desc_cur = -1;
     // eigensolver workspace
     double *ews = nullptr;
     size_t ews_doubles = 0;
@@ -36,9 +44,8 @@ struct nrg_ctx {
     // grouped GEMM: partial tiles of split outputs
     double *gpart = nullptr;
     size_t gpart_tiles = 0;
-    // pinned host staging: descriptors, transfers (read / written by kernels
-    // directly), the eigensolver status
-    void *hdesc = nullptr;
+    // pinned host memory: gathers (written by kernels directly), the
+    // eigensolver status
     int *hstat = nullptr;
     double *hpin = nullptr;
     size_t hpin_doubles = 0;
@@ -88,6 +95,7 @@ struct PendingLaunch {
     double flops, bytes;
 };
 extern bool g_timing;
+extern int g_timing_only;  // kernel id timed alone, -1: all
 extern nrg_kernel_stat g_stats[K_COUNT];
 extern std::vector<PendingLaunch> g_pending;
 void timing_fold();
@@ -96,7 +104,8 @@ struct LaunchTimer {
     bool on;
     PendingLaunch p;
     hipStream_t s;
-    LaunchTimer(int kid, hipStream_t st, double flops, double bytes) : on(g_timing), s(st) {
+    LaunchTimer(int kid, hipStream_t st, double flops, double bytes)
+        : on(g_timing && (g_timing_only < 0 || g_timing_only == kid)), s(st) {
         if (!on) return;
         p.kid = kid;
         p.flops = flops;
@@ -112,8 +121,15 @@ struct LaunchTimer {
     }
 };
 
-// Copies `bytes` of host descriptors into the context's staging buffer
-// (waits for the previous user of the buffer first).
+// Waits for an event / for everything queued on the context's stream by
+// polling (the runtime's blocking waits may sleep well past completion,
+// which leaves the GPU idle until the host wakes up).
+nrg_status wait_event(nrg_ctx *c, hipEvent_t e);
+nrg_status wait_stream(nrg_ctx *c);
+
+// Copies `bytes` of host descriptors into the next slot of the context's
+// staging ring (waiting only if that slot's previous use is still queued);
+// the device copy is valid for the work queued until the next stage call.
 nrg_status stage(nrg_ctx *c, const void *host, size_t bytes, void **dev);
 
 // ---------------------------------------------------------------------------

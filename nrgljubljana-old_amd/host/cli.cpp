@@ -39,6 +39,7 @@ extern "C" int nrg_main(int argc, char **argv) {
         if (a == "-w") next();
         else if (a == "--dump-eigen") opt.dump_eigen = next();
         else if (a == "--stats") opt.stats = next();
+        else if (a == "--stats-light") opt.stats_light = true;
         else if (a == "--tables") opt.tables_dir = next();
         else if (a == "--device") opt.device = std::atoi(next().c_str());
         else if (a == "--quiet") opt.quiet = true;
@@ -46,7 +47,7 @@ extern "C" int nrg_main(int argc, char **argv) {
         else if (a == "--hbm-budget") opt.hbm_budget_gb = std::atof(next().c_str());
         else if (a == "--spill-all") opt.hbm_budget_gb = 0.0;
         else if (a == "-h" || a == "--help") {
-            std::cout << "usage: nrg [-w workdir] [--dump-eigen file] [--stats file] [--tables dir] [--device n] [--quiet] [--plain-diag] [--hbm-budget GB] [--spill-all]\n";
+            std::cout << "usage: nrg [-w workdir] [--dump-eigen file] [--stats file [--stats-light]] [--tables dir] [--device n] [--quiet] [--plain-diag] [--hbm-budget GB] [--spill-all]\n";
             return 0;
         } else nrg::die("unknown option " + a);
     }

@@ -18,14 +18,55 @@
 #include <set>
 #include <sstream>
 
+#include <mutex>
+
 #include "nrg_host.h"
 
 namespace nrg {
 
+// exp(-x) in long double for x >= 0.  Past x = 11400 the result is below
+// half the smallest subnormal and rounds to zero, so it is returned without
+// calling expl (the Boltzmann factors of all but the last shells are there).
+static inline long double expl_neg(long double x) { return x > 11400.0L ? 0.0L : expl(-x); }
+
 // ---------------------------------------------------------------------------
 // device helpers
 // ---------------------------------------------------------------------------
-Gpu::Gpu(int device) { check(nrg_ctx_create(&ctx, device), "nrg_ctx_create"); }
+Gpu::Gpu(int dev) : device(dev) { check(nrg_ctx_create(&ctx, dev), "nrg_ctx_create"); }
+size_t Gpu::idle_bytes() const {
+    size_t b = 0;
+    for (auto &kv : pool_free) b += kv.first * sizeof(double);
+    return b;
+}
+
+namespace {
+std::mutex g_lease_m;
+// never destroyed: contexts must not be torn down during static destruction
+std::vector<std::unique_ptr<Gpu>> *g_idle = new std::vector<std::unique_ptr<Gpu>>();
+constexpr size_t kIdleMax = 2;
+}  // namespace
+
+GpuLease::GpuLease(int device) {
+    {
+        std::lock_guard<std::mutex> lk(g_lease_m);
+        for (auto it = g_idle->begin(); it != g_idle->end(); ++it)
+            if ((*it)->device == device) {
+                g = std::move(*it);
+                g_idle->erase(it);
+                break;
+            }
+    }
+    if (!g) g.reset(new Gpu(device));
+}
+GpuLease::~GpuLease() {
+    if (!g) return;
+    std::unique_ptr<Gpu> drop;
+    {
+        std::lock_guard<std::mutex> lk(g_lease_m);
+        if (g_idle->size() < kIdleMax) g_idle->push_back(std::move(g));
+        else drop = std::move(g);
+    }
+}
 Gpu::~Gpu() {
     if (ctx) (void)nrg_sync(ctx);
     for (auto &kv : pool_size) (void)nrg_dev_free(ctx, kv.first);
@@ -163,7 +204,8 @@ struct Engine {
     Params P;
     Data D;
     Tables T;
-    Gpu gpu;
+    GpuLease lease;  // before every member that allocates from it
+    Gpu &gpu;
     int Nmax = 0, combs = 0, nch = 0;
     std::vector<Iter> iters;
     double GSenergy = 0.0, totalenergy2 = 0.0, scale_last = 0.0;
@@ -198,9 +240,19 @@ struct Engine {
     int nbins = 0;
     double log10emin = 0, log10emax = 0;
     std::vector<double> bin_e;
+    // histograms of pass 2: one slot of 2 nbins per (shell, spectrum), read
+    // back once at the end and merged in the order they were filled
     double *dbins = nullptr;
+    struct BinSlot {
+        int N;
+        bool last;
+        size_t spec;
+        double scale;
+    };
+    std::vector<BinSlot> bin_slots;
+    size_t bin_capacity = 0;
 
-    explicit Engine(const Options &o) : opt(o), gpu(o.device) {}
+    explicit Engine(const Options &o) : opt(o), lease(o.device), gpu(*lease.g) {}
 
     double SCALE(int N) const {
         double s = 0.5 * (1.0 + 1.0 / P.Lambda);
@@ -210,6 +262,10 @@ struct Engine {
 
     template <class F>
     void timed(const char *name, F f) {
+        if (opt.stats_light) {  // no stage timing: keep the stream running
+            f();
+            return;
+        }
         auto t0 = std::chrono::steady_clock::now();
         f();
         gpu.check(nrg_sync(gpu.ctx), "nrg_sync");
@@ -227,6 +283,8 @@ struct Engine {
         {
             size_t fr = 0, tot = 0;
             gpu.check(nrg_mem_info(gpu.ctx, &fr, &tot), "nrg_mem_info");
+            // the pool's idle blocks (kept from an earlier solve) are available too
+            fr += gpu.idle_bytes();
             budget = opt.hbm_budget_gb >= 0.0 ? (size_t)(opt.hbm_budget_gb * 1e9) : (size_t)(0.6 * (double)fr);
         }
         nch = D.channels;
@@ -285,7 +343,6 @@ struct Engine {
             s.pos.assign(nbins, 0.0);
             s.neg.assign(nbins, 0.0);
         }
-        if (!spectra.empty()) dbins = gpu.alloc(2 * (size_t)nbins);
     }
 
     // ------------------------------------------------------------ state
@@ -502,14 +559,21 @@ struct Engine {
         fcustom << '\n';
     }
 
+    // A row of the custom file (calc_trace_singlet, src/nrg.cc:1204-1222):
+    // the diagonals of the singlet operators are collected on the device now
+    // (the operators are replaced by the next recalculation) and the rows are
+    // written at the end of the pass, so that the pass never waits for them.
+    struct CustomRow {
+        double Tnow;
+        std::map<Invar, std::vector<double>> vals;
+        double *dg;  // device copy of the diagonals
+        size_t n;
+    };
+    std::vector<CustomRow> custom_rows;
+    std::unique_ptr<Arena> custom_arena;
+
     void custom_row(double Tnow, const std::map<Invar, std::vector<double>> &vals,
                     const std::map<std::string, DevOp> &ops) {
-        double Z = 0;
-        for (auto &kv : vals)
-            for (double x : kv.second) Z += T.mult(kv.first) * std::exp(-P.betabar * x);
-        auto fo = [&](auto x) { fcustom << std::setw(P.width_custom) << std::setprecision(P.prec_custom) << x << ' '; };
-        fcustom << ' ';
-        fo(Tnow);
         std::vector<nrg_strided> items;
         size_t tot = 0;
         for (auto &name : singlet_names) {
@@ -523,22 +587,46 @@ struct Engine {
                 tot += n;
             }
         }
-        std::vector<double> dg(tot);
-        gpu.gather(items, dg.data());
-        size_t pos = 0;
-        for (size_t k = 0; k < singlet_names.size(); ++k) {
-            double tr = 0;
-            for (auto &kv : vals) {
-                const int n = (int)kv.second.size();
-                if (!n) continue;
-                double s = 0;
-                for (int r = 0; r < n; ++r) s += std::exp(-P.betabar * kv.second[r]) * dg[pos + r];
-                pos += n;
-                tr += T.mult(kv.first) * s;
+        if (!custom_arena) custom_arena.reset(new Arena(&gpu));
+        double *dg = tot ? custom_arena->alloc(tot) : nullptr;
+        if (tot) gpu.check(nrg_gather_dev(gpu.ctx, items.data(), (int)items.size(), dg), "nrg_gather_dev");
+        custom_rows.push_back({Tnow, vals, dg, tot});
+    }
+
+    void flush_custom() {
+        std::vector<nrg_strided> items;
+        size_t tot = 0;
+        for (auto &r : custom_rows)
+            if (r.n) {
+                items.push_back({r.dg, (int64_t)r.n, 1});
+                tot += r.n;
             }
-            fo(tr / Z);
+        std::vector<double> all(tot);
+        gpu.gather(items, all.data());
+        size_t pos = 0;
+        auto fo = [&](auto x) { fcustom << std::setw(P.width_custom) << std::setprecision(P.prec_custom) << x << ' '; };
+        for (auto &r : custom_rows) {
+            double Z = 0;
+            for (auto &kv : r.vals)
+                for (double x : kv.second) Z += T.mult(kv.first) * std::exp(-P.betabar * x);
+            fcustom << ' ';
+            fo(r.Tnow);
+            for (size_t k = 0; k < singlet_names.size(); ++k) {
+                double tr = 0;
+                for (auto &kv : r.vals) {
+                    const int n = (int)kv.second.size();
+                    if (!n) continue;
+                    double s = 0;
+                    for (int i = 0; i < n; ++i) s += std::exp(-P.betabar * kv.second[i]) * all[pos + i];
+                    pos += n;
+                    tr += T.mult(kv.first) * s;
+                }
+                fo(tr / Z);
+            }
+            fcustom << '\n';
         }
-        fcustom << '\n';
+        custom_rows.clear();
+        custom_arena.reset();
     }
 
     // ------------------------------------------------------------ recalc
@@ -953,32 +1041,47 @@ struct Engine {
         scale_last = SCALE(Nmax);
         if (!opt.quiet) std::cout << "Total energy: " << std::setprecision(16) << totalenergy << std::endl;
         ftd.close();
+        flush_custom();
         fcustom.close();
     }
 
     // ------------------------------------------------------------ DM sweeps
-    void rho_trace_check(const RhoMap &r, double expected, const char *what, int N) {
+    // Trace checks of density matrices (src/dmnrg.h:398-412), several at
+    // once: one transfer of all diagonals, then the checks in the given order.
+    struct TraceCheck {
+        const RhoMap *r;
+        double expected;
+        const char *what;
+        int N;
+    };
+    void rho_trace_check_all(const std::vector<TraceCheck> &checks) {
         std::vector<nrg_strided> items;
         size_t tot = 0;
-        for (auto &kv : r) {
-            if (!kv.second.dim) continue;
-            items.push_back({kv.second.p, kv.second.dim, kv.second.diag ? 1 : kv.second.dim + 1});
-            tot += kv.second.dim;
-        }
+        for (auto &c : checks)
+            for (auto &kv : *c.r) {
+                if (!kv.second.dim) continue;
+                items.push_back({kv.second.p, kv.second.dim, kv.second.diag ? 1 : kv.second.dim + 1});
+                tot += kv.second.dim;
+            }
         std::vector<double> dg(tot);
         gpu.gather(items, dg.data());
-        double tr = 0;
         size_t pos = 0;
-        for (auto &kv : r) {
-            if (!kv.second.dim) continue;
-            double s = 0;
-            for (int i = 0; i < kv.second.dim; ++i) s += dg[pos + i];
-            pos += kv.second.dim;
-            tr += T.mult(kv.first) * s;
+        for (auto &c : checks) {
+            double tr = 0;
+            for (auto &kv : *c.r) {
+                if (!kv.second.dim) continue;
+                double s = 0;
+                for (int i = 0; i < kv.second.dim; ++i) s += dg[pos + i];
+                pos += kv.second.dim;
+                tr += T.mult(kv.first) * s;
+            }
+            const double rel = (tr - c.expected) / c.expected;
+            if (!(std::fabs(rel) < 1e-8))
+                die(std::string(c.what) + " trace check failed at N=" + std::to_string(c.N) + ": " + std::to_string(tr));
         }
-        const double rel = (tr - expected) / expected;
-        if (!(std::fabs(rel) < 1e-8))
-            die(std::string(what) + " trace check failed at N=" + std::to_string(N) + ": " + std::to_string(tr));
+    }
+    void rho_trace_check(const RhoMap &r, double expected, const char *what, int N) {
+        rho_trace_check_all({TraceCheck{&r, expected, what, N}});
     }
 
     // calc_densitymatrix_iterN / calc_fulldensitymatrix_iterN with cdmI
@@ -1066,7 +1169,7 @@ struct Engine {
                 for (int i = mn; i < kv.second.d; ++i) {
                     const long double Eabs = (long double)kv.second.absenergy[i] - (long double)GSenergy;
                     if (Eabs < 0) die("calc_ZnD: negative excitation energy");
-                    ZZ += T.mult(kv.first) * expl(-Eabs / (long double)P.T);
+                    ZZ += T.mult(kv.first) * expl_neg(Eabs / (long double)P.T);
                 }
             }
             ZnD[N] = (double)ZZ;
@@ -1097,7 +1200,7 @@ struct Engine {
                 double hv = 0.0;
                 if (i >= mn) {
                     const long double Eabs = (long double)kv.second.absenergy[i] - (long double)GSenergy;
-                    long double v = expl(-Eabs / (long double)P.T) * ((long double)wn[N] / (long double)ZnD[N]);
+                    long double v = expl_neg(Eabs / (long double)P.T) * ((long double)wn[N] / (long double)ZnD[N]);
                     if (!std::isfinite((double)v)) v = 0;
                     hv = (double)v;
                 }
@@ -1122,19 +1225,24 @@ struct Engine {
         init_rho();
         if (P.dmnrg) {
             timed("DM", [&] {
+                // the matrices stay resident: their traces are checked after
+                // the sweep, so that the sweep never waits for the host
+                std::vector<TraceCheck> checks;
                 RhoMap cur = rho_init;
                 for (int N = Nmax - 1; N > P.Ninit; --N) {
                     use(N);
                     Arena tmp(&gpu);
                     RhoMap prev = dm_step(N, cur, nullptr, *rho_arena, tmp);
-                    rho_trace_check(prev, 1.0, "DM", N);
                     rhos[N - 1] = prev;
+                    checks.push_back({&rhos[N - 1], 1.0, "DM", N});
                     cur = prev;
                 }
+                rho_trace_check_all(checks);
             });
         }
         if (P.fdm) {
             timed("FDM", [&] {
+                std::vector<TraceCheck> checks;
                 RhoMap cur = init_rho_FDM(Nmax - 1);
                 for (int N = Nmax - 1; N > P.Ninit; --N) {
                     use(N);
@@ -1145,10 +1253,11 @@ struct Engine {
                     RhoMap prev = dm_step(N, cur, last ? nullptr : &dd, *rho_arena, tmp);
                     double expected = 0;
                     for (int n = N; n < Nmax; ++n) expected += wn[n];
-                    rho_trace_check(prev, expected, "FDM", N);
                     rhofdm[N - 1] = prev;
+                    checks.push_back({&rhofdm[N - 1], expected, "FDM", N});
                     cur = prev;
                 }
+                rho_trace_check_all(checks);
             });
         }
     }
@@ -1271,25 +1380,46 @@ struct Engine {
             const double fac = getfactor();
             prm.emin_w = P.goodE;
             prm.emax_w = P.goodE * fac * fac;
-            gpu.check(nrg_zero(gpu.ctx, dbins, 2 * (size_t)nbins), "nrg_zero");
-            gpu.check(nrg_spec_bin(gpu.ctx, &prm, pairs.data(), (int)pairs.size(), dbins), "nrg_spec_bin");
-            std::vector<double> hb(2 * (size_t)nbins);
-            gpu.gather({nrg_strided{dbins, (int64_t)hb.size(), 1}}, hb.data());
-            std::vector<double> pos(hb.begin(), hb.begin() + nbins), neg(hb.begin() + nbins, hb.end());
-            if (fdm) {
-                for (int k = 0; k < nbins; ++k) {
-                    sp.pos[k] += pos[k];
-                    sp.neg[k] += neg[k];
+            if (bin_slots.size() >= bin_capacity) die("spectra: more histogram slots than planned");
+            double *slot = dbins + bin_slots.size() * 2 * (size_t)nbins;
+            gpu.check(nrg_spec_bin(gpu.ctx, &prm, pairs.data(), (int)pairs.size(), slot), "nrg_spec_bin");
+            bin_slots.push_back({N, last, (size_t)(&sp - spectra.data()), it.scale});
+        }
+    }
+
+    // Reads all histograms of pass 2 at once and merges them into the
+    // spectra in the order they were filled (as the reference does shell by
+    // shell: summed for FDM, merge_nn2 for DM-NRG).
+    void merge_bins() {
+        if (bin_slots.empty()) return;
+        const size_t w = 2 * (size_t)nbins;
+        std::vector<double> hb(bin_slots.size() * w);
+        gpu.gather({nrg_strided{dbins, (int64_t)hb.size(), 1}}, hb.data());
+        for (size_t k = 0; k < bin_slots.size(); ++k) {
+            const BinSlot &b = bin_slots[k];
+            Spec &sp = spectra[b.spec];
+            std::vector<double> pos(hb.begin() + k * w, hb.begin() + k * w + nbins),
+                neg(hb.begin() + k * w + nbins, hb.begin() + (k + 1) * w);
+            if (sp.kind == "FDM") {
+                for (int i = 0; i < nbins; ++i) {
+                    sp.pos[i] += pos[i];
+                    sp.neg[i] += neg[i];
                 }
             } else {
-                merge_nn2(N, last, sp, pos, neg, it.scale);
+                merge_nn2(b.N, b.last, sp, pos, neg, b.scale);
             }
         }
+        bin_slots.clear();
     }
 
     void pass2() {
         State st;
         fresh_state(st);
+        if (!spectra.empty()) {
+            bin_capacity = (size_t)std::max(0, Nmax - P.Ninit) * spectra.size();
+            dbins = gpu.alloc(std::max<size_t>(1, bin_capacity * 2 * (size_t)nbins));
+            gpu.check(nrg_zero(gpu.ctx, dbins, bin_capacity * 2 * (size_t)nbins), "nrg_zero");
+        }
         double totalenergy = D.GS;
         const bool recalc_all = P.fdm;  // do_recalc_all (src/nrg.cc:2436-2444)
         for (int N = P.Ninit; N < Nmax; ++N) {
@@ -1334,6 +1464,7 @@ struct Engine {
             }
             st.old_op_arena.reset();
         }
+        merge_bins();
         totalenergy2 = totalenergy;
         if (std::fabs(totalenergy2 - GSenergy) > 1e-10 * std::max(1.0, std::fabs(GSenergy)))
             die("GS energy differs between the two passes");

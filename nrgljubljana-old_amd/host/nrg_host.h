@@ -86,6 +86,7 @@ struct Tables {
 // users.
 struct Gpu {
     nrg_ctx *ctx = nullptr;
+    int device = 0;
     std::multimap<size_t, double *> pool_free;
     std::map<double *, size_t> pool_size;
     explicit Gpu(int device);
@@ -100,6 +101,21 @@ struct Gpu {
     void scatter(const std::vector<nrg_strided> &items, const double *host);
     // returns the pool's unused blocks to the device (after a spill)
     void trim();
+    // bytes held by the pool and not in use
+    size_t idle_bytes() const;
+};
+
+// A device context with its memory pool, lent to one solve at a time.  On
+// release the context (streams, staging, workspaces) and the pool's blocks
+// are kept for the next solve in the process (a few are kept; solves in a
+// serving loop reuse them), so repeated solves skip stream creation and the
+// device-synchronising allocations and frees.
+struct GpuLease {
+    std::unique_ptr<Gpu> g;
+    explicit GpuLease(int device);
+    ~GpuLease();
+    GpuLease(const GpuLease &) = delete;
+    GpuLease &operator=(const GpuLease &) = delete;
 };
 
 // Bump allocator over large device chunks, released together.
@@ -162,6 +178,7 @@ struct Options {
     std::string workdir = ".";
     std::string dump_eigen;  // binary dump of eigenvalues / kept counts
     std::string stats;       // JSON with stage timings
+    bool stats_light = false;  // stats without stage timing (no stream syncs between stages)
     std::string tables_dir;
     int device = 0;
     bool quiet = false;  // no per-iteration log lines
