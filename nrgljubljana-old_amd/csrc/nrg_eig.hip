@@ -379,14 +379,21 @@ constexpr int REG_ONE_N = 128;  // one workgroup: 16 waves x 8 rows, 2 column re
 constexpr int REG_MAX_N = 576;  // largest team variant: 9 column registers
 constexpr int REG_NT = 512;     // team workgroups: 8 waves
 
+// x[col >> 6] of lane col & 63 (col the same in every lane), as a scalar: a
+// chain of uniform branches, each reading one fixed register (a select over
+// the registers would be turned into an indexed array in memory)
+template <int NC, int K = 0>
+__device__ __forceinline__ double reg_pick(const double (&x)[NC], int i, int lane) {
+    if constexpr (K == NC - 1) {
+        return readlane_d(x[K], lane);
+    } else {
+        if (i == K) return readlane_d(x[K], lane);
+        return reg_pick<NC, K + 1>(x, i, lane);
+    }
+}
 template <int NC>
 __device__ __forceinline__ double reg_at(const double (&x)[NC], int col) {
-    const int i = col >> 6;  // the same in every lane
-    double v = x[0];
-#pragma unroll
-    for (int k = 1; k < NC; ++k)
-        if (i == k) v = x[k];
-    return __shfl(v, col & 63, 64);
+    return reg_pick<NC>(x, __builtin_amdgcn_readfirstlane(col >> 6), __builtin_amdgcn_readfirstlane(col & 63));
 }
 
 // team size of the register variant for dimension n (REG_ONE_N < n <=
@@ -396,18 +403,12 @@ __host__ __device__ inline int reg_team(int n) {
     return (n + rows - 1) / rows;
 }
 
-// Butterfly sums of R values at once (each leaves the same bits in every
-// lane, as wave_sum); the R shuffle chains are independent and overlap.
+// Wave sums of R values at once (as wave_sum: the same bits in every lane);
+// the R chains are independent and overlap.
 template <int R>
 __device__ __forceinline__ void wave_sum_n(double (&v)[R]) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        double t[R];
-#pragma unroll
-        for (int s = 0; s < R; ++s) t[s] = __shfl_xor(v[s], o, 64);
-#pragma unroll
-        for (int s = 0; s < R; ++s) v[s] += t[s];
-    }
+    for (int s = 0; s < R; ++s) v[s] = wave_sum(v[s]);
 }
 
 template <int NC, int RPW, int NW, bool TEAM>

@@ -135,9 +135,29 @@ nrg_status stage(nrg_ctx *c, const void *host, size_t bytes, void **dev);
 // ---------------------------------------------------------------------------
 // reductions (wave64)
 // ---------------------------------------------------------------------------
+// DPP move of a double within rows of 16 lanes (two 32-bit moves on the VALU)
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+// Lane `lane` (the same in every lane) of a double, as a scalar
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+// Sum over the 64 lanes of a wavefront (called with every lane active): pairs
+// within quads and rows by DPP (quad_perm [1,0,3,2], [2,3,0,1], row half
+// mirror, row mirror), then the four row sums read as scalars and added in a
+// fixed order, so every lane returns the same bits.  No LDS traffic, unlike
+// shuffles, which all go through the CU's LDS unit.
 __device__ inline double wave_sum(double v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += dpp_d<0xB1>(v);
+    v += dpp_d<0x4E>(v);
+    v += dpp_d<0x141>(v);
+    v += dpp_d<0x140>(v);
+    return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 __device__ inline double wave_max(double v) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
