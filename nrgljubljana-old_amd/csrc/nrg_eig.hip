@@ -19,9 +19,11 @@
 //                and multiplies them into the merged block with the grouped
 //                GEMM.  Eigenvectors of the tridiagonal are kept column-major
 //                (one contiguous row per eigenvector).
-//   k_eig_bt     back-transformation: one wavefront per eigenvector applies
-//                the reflectors H_{n-3} .. H_0 with the vector held in
-//                registers, and writes it as row rank(c) of the result.
+//   k_eig_bt     back-transformation (n < 1025): one wavefront per eigenvector
+//                applies the reflectors H_{n-3} .. H_0 (staged eight at a time
+//                in LDS, shared by the eight eigenvectors of a workgroup) with
+//                the vector held in registers, and writes it as row rank(c)
+//                of the result; larger n: compact WY blocks and GEMMs.
 // Output convention as diagonalise(): eigenvalues ascending in W, row r of A
 // = eigenvector r.
 #include "nrg_internal.h"
@@ -1304,39 +1306,52 @@ __global__ __launch_bounds__(256) void k_dc_rank(const EigMat *mats, const BtIte
 //    vector in registers (NS x 64 elements), reflectors read from A.
 //    Output: row rank(c) of S1.
 // ---------------------------------------------------------------------------
+// BT_NT threads: eight eigenvectors per workgroup, one wavefront each; the
+// reflectors are staged BT_RB at a time in LDS (read once per workgroup
+// instead of once per eigenvector) and applied from there.
+constexpr int BT_NT = 512, BT_RB = 8;
 template <int NS>
-__global__ __launch_bounds__(256) void k_eig_bt(const EigMat *mats, const BtItem *items) {
+__global__ __launch_bounds__(BT_NT) void k_eig_bt(const EigMat *mats, const BtItem *items) {
+    __shared__ double refl[BT_RB][64 * NS];
     const BtItem it = items[blockIdx.x];
     const EigMat mt = mats[it.mat];
     const int n = mt.n;
     const int c = it.c0 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-    if (c >= n) return;
+    const bool act = c < n;  // the same in the whole wavefront
     double *ws = mt.ws;
-    const double *xc = ws + (size_t)c * n;
     const double *tau = vslot(ws, n, V_TAU);
     const double *A = mt.A;
     double x[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
         const int r = 64 * k + lane;
-        x[k] = (r < n) ? xc[r] : 0.0;
+        x[k] = (act && r < n) ? ws[(size_t)c * n + r] : 0.0;
     }
-    for (int j = n - 3; j >= 0; --j) {
-        const double t = tau[j];
-        if (t == 0.0) continue;
-        const double *v = A + (size_t)j * n;
-        double vr[NS];
-        double dot = 0.0;
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-            const int r = 64 * k + lane;
-            vr[k] = (r > j && r < n) ? v[r] : 0.0;
-            dot += vr[k] * x[k];
+    for (int j0 = n - 3; j0 >= 0; j0 -= BT_RB) {
+        const int nb = min(BT_RB, j0 + 1);  // reflectors j0, j0-1, .., j0-nb+1
+        __syncthreads();                    // the previous batch has been applied
+        for (int e = threadIdx.x; e < nb * 64 * NS; e += BT_NT) {
+            const int b = e / (64 * NS), r = e - b * 64 * NS, j = j0 - b;
+            refl[b][r] = (r > j && r < n) ? A[(size_t)j * n + r] : 0.0;
         }
-        dot = wave_sum(dot) * t;
+        __syncthreads();
+        if (!act) continue;
+        for (int b = 0; b < nb; ++b) {
+            const double t = tau[j0 - b];
+            if (t == 0.0) continue;
+            double vr[NS];
+            double dot = 0.0;
 #pragma unroll
-        for (int k = 0; k < NS; ++k) x[k] -= dot * vr[k];
+            for (int k = 0; k < NS; ++k) {
+                vr[k] = refl[b][64 * k + lane];
+                dot += vr[k] * x[k];
+            }
+            dot = wave_sum(dot) * t;
+#pragma unroll
+            for (int k = 0; k < NS; ++k) x[k] -= dot * vr[k];
+        }
     }
+    if (!act) return;
     const int rk = islot(ws, n, V_RANK)[c];
     double *out = ws + (size_t)n * n + (size_t)rk * n;
 #pragma unroll
@@ -1428,12 +1443,13 @@ __global__ __launch_bounds__(256) void k_eig_copy(const EigMat *mats, const BtIt
     for (size_t e = e0 + threadIdx.x; e < e1; e += 256) mt.A[e] = src[e];
 }
 
-constexpr int kBtSlots[] = {1, 2, 4, 8, 12, 16, 24, 32, 48, 64};
+constexpr int kBtSlots[] = {1, 2, 4, 8, 12, 16};  // 64 x 16 columns >= BT_WY_MIN - 1
 constexpr int kBtVariants = sizeof(kBtSlots) / sizeof(kBtSlots[0]);
+static_assert(64 * kBtSlots[kBtVariants - 1] >= BT_WY_MIN - 1, "register back-transformation covers every n below WY");
 
 template <int NS>
 static void launch_bt(hipStream_t s, int nitems, const EigMat *mats, const BtItem *items) {
-    hipLaunchKernelGGL(k_eig_bt<NS>, dim3(nitems), dim3(256), 0, s, mats, items);
+    hipLaunchKernelGGL(k_eig_bt<NS>, dim3(nitems), dim3(BT_NT), 0, s, mats, items);
 }
 static void launch_bt_variant(int v, hipStream_t s, int nitems, const EigMat *mats, const BtItem *items) {
     switch (v) {
@@ -1442,11 +1458,7 @@ static void launch_bt_variant(int v, hipStream_t s, int nitems, const EigMat *ma
         case 2: launch_bt<4>(s, nitems, mats, items); break;
         case 3: launch_bt<8>(s, nitems, mats, items); break;
         case 4: launch_bt<12>(s, nitems, mats, items); break;
-        case 5: launch_bt<16>(s, nitems, mats, items); break;
-        case 6: launch_bt<24>(s, nitems, mats, items); break;
-        case 7: launch_bt<32>(s, nitems, mats, items); break;
-        case 8: launch_bt<48>(s, nitems, mats, items); break;
-        default: launch_bt<64>(s, nitems, mats, items); break;
+        default: launch_bt<16>(s, nitems, mats, items); break;
     }
 }
 
@@ -1580,7 +1592,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         } else {
             int v = 0;
             while (kBtSlots[v] < ns) ++v;
-            for (int c0 = 0; c0 < n; c0 += 4) bt[v].push_back({i, c0});
+            for (int c0 = 0; c0 < n; c0 += BT_NT / 64) bt[v].push_back({i, c0});
             const size_t nn = (size_t)n * n;
             for (size_t e = 0; e < nn; e += 256 * 16) cp.push_back({i, (int)(e / (256 * 16))});
         }
