@@ -472,14 +472,34 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
             for (int s = 0; s < RPW; ++s) wrow[s] = 0.0;
         } else {
             const double *pg = ((j + 2) & 1) ? pb1 : pb0, *rg = ((j + 2) & 1) ? rb1 : rb0;
+            if constexpr (TEAM) {
+                // the workgroup reads the published p_j and row j+1 (columns
+                // j+1..n-1) once into LDS, rather than every wavefront from L2
+                constexpr int NL = (2 * NC + NW - 1) / NW;
+                const int m = n - j - 1;
+                double t[NL];
+#pragma unroll
+                for (int u = 0; u < NL; ++u) {
+                    const int k = min(tid + 64 * NW * u, 2 * m - 1);
+                    t[u] = k < m ? ld_dev(pg + j + 1 + k) : ld_dev(rg + j + 1 + k - m);
+                }
+#pragma unroll
+                for (int u = 0; u < NL; ++u) {
+                    const int k = tid + 64 * NW * u;
+                    if (k < 2 * m) lbuf[k < m ? j + 1 + k : n + j + 1 + k - m] = t[u];
+                }
+                __syncthreads();
+                pg = lbuf;
+                rg = lbuf + n;
+            }
             double pp = 0.0;
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
                 const int c = lane + 64 * i;
                 const bool ok = c > j && c < n;
                 const unsigned cc = (unsigned)min(c, n - 1);
-                const double pl = TEAM ? ld_dev(pg + cc) : pg[cc];
-                const double rl = TEAM ? ld_dev(rg + cc) : rg[cc];
+                const double pl = pg[cc];
+                const double rl = rg[cc];
                 w[i] = ok ? pl : 0.0;
                 xr[i] = ok ? rl : 0.0;
                 pp += vv[i] * w[i];
@@ -609,6 +629,7 @@ __global__ __launch_bounds__(TRI_NT) void k_eig_tri_one(const EigMat *mats, cons
 __global__ __launch_bounds__(REG_NT) void k_eig_tri_reg(const EigMat *mats, const TriJob *jobs, const int *tbase,
                                                         const int *qjob, int *counters, int *status) {
     __shared__ int s_ticket, s_flag;
+    __shared__ double lbuf[2 * REG_MAX_N];
     const int x = blockIdx.x % N_XCD;
     int *ticket = counters + x;
     int *bars = counters + N_XCD + 1;
@@ -629,9 +650,9 @@ __global__ __launch_bounds__(REG_NT) void k_eig_tri_reg(const EigMat *mats, cons
         const TriJob jb = jobs[lo];
         const EigMat mt = mats[jb.mat];
         const int rank = t0 + t - tbase[lo];
-        if (mt.n <= 256) tri_reg<4, 8, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, nullptr);
-        else if (mt.n <= 384) tri_reg<6, 4, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, nullptr);
-        else tri_reg<9, 3, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, nullptr);
+        if (mt.n <= 256) tri_reg<4, 8, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf);
+        else if (mt.n <= 384) tri_reg<6, 4, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf);
+        else tri_reg<9, 3, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf);
         __syncthreads();
     }
 }
