@@ -765,10 +765,12 @@ __global__ __launch_bounds__(256) void k_dc_leaf(const EigMat *mats) {
 // value), so the carried element stays in a register until the next new
 // pole replaces it; a segment starts at a new pole (or at q = 0), so no
 // carry crosses segments.
-__device__ inline void dc_gather_seg(double *ws, int n, int a, int h, int q0, int q1, int r, const int *perm,
-                                     const int *pos, const int *kind, const double *rc, const double *rs) {
-    const double *Vc = ws + (size_t)a * n + a;
-    double *G = ws + (size_t)n * n + (size_t)a * n + a;
+// Vc and G point at the merge's diagonal block (row and column a), both with
+// row stride ld.
+__device__ inline void dc_gather_seg(const double *Vc, double *G, int ld, int h, int q0, int q1, int r,
+                                     const int *perm, const int *pos, const int *kind, const double *rc,
+                                     const double *rs) {
+    const size_t n = ld;
     const bool lo = r < h;
     int cr = -1;
     double cv = 0.0;
@@ -1112,7 +1114,8 @@ __global__ __launch_bounds__(256) void k_dc_gather(const DcMerge *merges, const 
     if (sgi >= nseg || r >= K) return;
     const int *seg = islot(ws, n, V_SEG) + a;
     const int q0 = max(0, seg[sgi]), q1 = min(K, sgi + 1 < nseg ? seg[sgi + 1] : K);
-    dc_gather_seg(ws, n, a, mg.mid - a, q0, q1, r, islot(ws, n, V_PERM) + a, islot(ws, n, V_POS) + a,
+    dc_gather_seg(ws + (size_t)a * n + a, ws + (size_t)n * n + (size_t)a * n + a, n, mg.mid - a, q0, q1, r,
+                  islot(ws, n, V_PERM) + a, islot(ws, n, V_POS) + a,
                   islot(ws, n, V_KIND) + a, vslot(ws, n, V_RC) + a, vslot(ws, n, V_RS) + a);
 }
 
@@ -1137,8 +1140,18 @@ __device__ inline void dc_copy_deflated(const DcMerge &mg, int j0) {
 // c + s / (pole - x) matching value and slope at the current point; the
 // model's root (a quadratic) is taken if it lies inside the bracket,
 // otherwise the bracket is bisected.
+// W lanes share the sums over i: a wavefront (64; lane = 0..63) or a row of
+// 16 lanes (16; lane = 0..15 within the row, every lane of the row active).
+template <int W = 64>
+__device__ inline double lanes_sum(double v) {
+    if constexpr (W == 64) return wave_sum(v);
+    else return row_sum(v);
+}
+template <int W = 64>
 __device__ inline void secular_root(const double *dd, const double *zz, int K, double rho, int j, int lane,
                                     int &org_out, double &tau_out) {
+    static_assert(W == 64 || W == 16, "a wavefront or a row of 16 lanes");
+    const int i0 = lane, istep = W;
     const double irho = 1.0 / rho;
     if (K == 1) {
         org_out = 0;
@@ -1152,8 +1165,8 @@ __device__ inline void secular_root(const double *dd, const double *zz, int K, d
         const double dj = dd[j];
         const double mid = 0.5 * (dd[j + 1] - dj);
         double f = 0.0;
-        for (int i = lane; i < K; i += 64) f += zz[i] * zz[i] / ((dd[i] - dj) - mid);
-        f = irho + wave_sum(f);
+        for (int i = i0; i < K; i += istep) f += zz[i] * zz[i] / ((dd[i] - dj) - mid);
+        f = irho + lanes_sum<W>(f);
         if (f >= 0.0) {
             org = j;
             lo = 0.0;
@@ -1165,8 +1178,8 @@ __device__ inline void secular_root(const double *dd, const double *zz, int K, d
         }
     } else {
         double zt = 0.0;
-        for (int i = lane; i < K; i += 64) zt += zz[i] * zz[i];
-        zt = wave_sum(zt);
+        for (int i = i0; i < K; i += istep) zt += zz[i] * zz[i];
+        zt = lanes_sum<W>(zt);
         org = j;
         lo = 0.0;
         hi = rho * zt;
@@ -1177,7 +1190,7 @@ __device__ inline void secular_root(const double *dd, const double *zz, int K, d
     double tau = 0.5 * (lo + hi);
     for (int it = 0; it < 64; ++it) {
         double psi = 0.0, dpsi = 0.0, phi = 0.0, dphi = 0.0;
-        for (int i = lane; i < K; i += 64) {
+        for (int i = i0; i < K; i += istep) {
             const double r = 1.0 / ((dd[i] - dorg) - tau);
             const double t = zz[i] * zz[i] * r;
             if (i <= j) {
@@ -1188,10 +1201,10 @@ __device__ inline void secular_root(const double *dd, const double *zz, int K, d
                 dphi += t * r;
             }
         }
-        psi = wave_sum(psi);
-        dpsi = wave_sum(dpsi);
-        phi = wave_sum(phi);
-        dphi = wave_sum(dphi);
+        psi = lanes_sum<W>(psi);
+        dpsi = lanes_sum<W>(dpsi);
+        phi = lanes_sum<W>(phi);
+        dphi = lanes_sum<W>(dphi);
         const double f = irho + psi + phi;
         const double err = 8.0 * (fabs(psi) + fabs(phi) + irho) + fabs(tau) * (dpsi + dphi);
         if (f < 0.0) lo = tau;
@@ -1301,6 +1314,291 @@ __global__ __launch_bounds__(256) void k_dc_unorm(const DcMerge *merges, const D
     s = wave_sum(s);
     const double inv = 1.0 / sqrt(s);
     for (int i = lane; i < kp; i += 64) row[i] = zh[i] / row[i] * inv;
+}
+
+// ---------------------------------------------------------------------------
+// 2b. the lowest levels in one launch.  One workgroup per subtree of the
+//     tree at level DC_SMALL_LEV (2^5 leaves of 1 or 2 rows: S <= 64 rows)
+//     runs levels 1..5 of that subtree in LDS, with block barriers between
+//     the stages instead of one launch per stage and level, and writes the
+//     subtree's eigenvalues and S x S eigenvector block back at the end.
+//     Every merge has K <= 64 poles, so one wavefront deflates a merge with
+//     lane q holding sorted pole q.  The arithmetic is that of the
+//     level-by-level kernels: sort by (value, index), z, the dlaed2
+//     deflation rule (parallel tests, then the candidates and rotation
+//     chains in order), the gather with the rotations, secular roots,
+//     z-hat, normalisation and the merge product U G.
+// ---------------------------------------------------------------------------
+constexpr int DC_SMALL_LEV = 5;
+constexpr int DC_SMALL_S = 64;   // rows of a subtree, at most
+constexpr int DC_SMALL_LD = 65;  // LDS row stride of the blocks (doubles)
+constexpr int DC_SMALL_NT = 1024, DC_SMALL_NW = DC_SMALL_NT / 64;
+
+struct DcSmall {
+    double V[DC_SMALL_S * DC_SMALL_LD];  // eigenvectors (rows), block-diagonal by merge
+    double G[DC_SMALL_S * DC_SMALL_LD];  // gathered rows of the current merges
+    double U[DC_SMALL_S * DC_SMALL_LD];  // Delta, then the eigenvectors of D + rho z z^T (rows)
+    double D[DC_SMALL_S], DD[DC_SMALL_S], ZZ[DC_SMALL_S], ZH[DC_SMALL_S];
+    double RC[DC_SMALL_S], RS[DC_SMALL_S], BETA[DC_SMALL_S], RHO[DC_SMALL_S];
+    int PERM[DC_SMALL_S], POS[DC_SMALL_S], KIND[DC_SMALL_S], ROWM[DC_SMALL_S];
+    int MA[DC_SMALL_S], MB[DC_SMALL_S], KP[DC_SMALL_S];
+};
+
+// local bounds of merge m of level L of subtree k0 (at level ls) starting at
+// global row a0
+__device__ inline void dc_small_bounds(int n, int nlev, int ls, int k0, int a0, int L, int m, int &a, int &mid,
+                                       int &b) {
+    const int k = (k0 << (ls - L)) + m;
+    a = dc_bound(k << L, n, nlev) - a0;
+    mid = dc_bound((k << L) + (1 << (L - 1)), n, nlev) - a0;
+    b = dc_bound((k + 1) << L, n, nlev) - a0;
+}
+
+// Deflation of local merge m = [a, b) (K = b - a <= 64) by one wavefront.
+__device__ inline void dc_small_deflate(DcSmall &sm, int m, int a, int mid, int b, int lane) {
+    const int K = b - a, h = mid - a;
+    const double beta = sm.BETA[mid];
+    const double rho = 2.0 * fabs(beta);
+    const double sg = beta >= 0.0 ? 1.0 : -1.0;
+    const double rs2 = 0.70710678118654752440;
+    const bool act = lane < K;
+    const double di = act ? sm.D[a + lane] : 0.0;
+    // rank of pole lane in the order (value, index); pm: the pole of rank lane
+    int rk = lane;
+    if (act) {
+        rk = 0;
+        for (int j = 0; j < K; ++j) {
+            const double dj = readlane_d(di, j);
+            rk += (dj < di) || (dj == di && j < lane);
+        }
+    }
+    int pm = lane;
+    for (int j = 0; j < K; ++j)
+        if (__builtin_amdgcn_readlane(rk, j) == lane) pm = j;
+    double ds = __shfl(di, pm, 64), zs = 0.0;
+    if (act) {
+        const double *row = sm.V + (a + pm) * DC_SMALL_LD;
+        zs = pm < h ? row[mid - 1] * rs2 : sg * row[mid] * rs2;
+    }
+    const double md = wave_max(fabs(ds)), mz = wave_max(fabs(zs));
+    const double tol = 8.0 * kEps * fmax(md, rho * mz);
+    const bool large = act && !(rho * fabs(zs) <= tol);
+    const unsigned long long Lm = __ballot(large);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    const unsigned long long Lb = Lm & below;
+    const int prev = Lb ? 63 - __builtin_clzll(Lb) : -1;
+    // tests with the unrotated values
+    const int src = prev < 0 ? lane : prev;
+    const double zp = __shfl(zs, src, 64), dp = __shfl(ds, src, 64);
+    bool cand = false;
+    if (large && prev >= 0) {
+        double c, s, tau;
+        cand = dc_close(zp, zs, dp, ds, tol, c, s, tau);
+    }
+    unsigned long long Cm = __ballot(cand);
+    int kind = large ? 1 : 0;
+    double rcv = 0.0, rsv = 0.0;
+    int chain = -1;
+    for (;;) {  // wave-uniform walk over the candidates and rotation chains
+        int q;
+        if (chain >= 0) {
+            q = chain;
+            chain = -1;
+            Cm &= ~((2ull << q) - 1ull);  // candidates up to q are settled
+        } else if (Cm) {
+            q = __builtin_ctzll(Cm);
+            Cm &= Cm - 1ull;
+        } else {
+            break;
+        }
+        const int p = __builtin_amdgcn_readlane(prev, q);
+        const double zpq = readlane_d(zs, p), zqq = readlane_d(zs, q);
+        const double dpq = readlane_d(ds, p), dqq = readlane_d(ds, q);
+        double c, s, tau;
+        if (dc_close(zpq, zqq, dpq, dqq, tol, c, s, tau)) {
+            const double tp = dpq * c * c + dqq * s * s, tq = dpq * s * s + dqq * c * c;
+            if (lane == q) {
+                zs = tau;
+                ds = tq;
+                kind = 2;
+                rcv = c;
+                rsv = s;
+            }
+            if (lane == p) {
+                zs = 0.0;
+                ds = tp;
+                kind |= KIND_SRC;
+            }
+            // the next large-z pole is tested against the rotated values
+            const unsigned long long above = Lm & ~((2ull << q) - 1ull);
+            if (above) chain = __builtin_ctzll(above);
+        }
+    }
+    // positions: undeflated poles first, then the deflated ones, both in
+    // sorted order
+    const bool und = (kind & KIND_MASK) != 0 && !(kind & KIND_SRC);
+    const unsigned long long Um = __ballot(und);
+    const int kp = __popcll(Um), ub = __popcll(Um & below);
+    const int ps = und ? ub : kp + (lane - ub);
+    if (act) {
+        sm.PERM[a + lane] = pm;
+        sm.POS[a + lane] = ps;
+        sm.KIND[a + lane] = kind;
+        sm.RC[a + lane] = rcv;
+        sm.RS[a + lane] = rsv;
+        sm.ROWM[a + lane] = m;
+        if (ps < kp) {
+            sm.DD[a + ps] = ds;
+            sm.ZZ[a + ps] = zs;
+        } else {
+            sm.D[a + ps] = ds;  // deflated eigenvalue, final for this merge
+        }
+    }
+    if (lane == 0) {
+        sm.MA[m] = a;
+        sm.MB[m] = b;
+        sm.KP[m] = kp;
+        sm.RHO[m] = rho;
+    }
+}
+
+__global__ __launch_bounds__(DC_SMALL_NT) void k_dc_small(const EigMat *mats, const BtItem *items) {
+    extern __shared__ double lds[];
+    DcSmall &sm = *reinterpret_cast<DcSmall *>(lds);
+    const BtItem it = items[blockIdx.x];
+    const EigMat mt = mats[it.mat];
+    const int n = mt.n, nlev = dc_levels(n), ls = min(DC_SMALL_LEV, nlev), k0 = it.c0;
+    double *ws = mt.ws;
+    double *Vg = ws, *Dg = vslot(ws, n, V_D);
+    const double *betag = vslot(ws, n, V_BETA);
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int a0 = dc_bound(k0 << ls, n, nlev), S = dc_bound((k0 + 1) << ls, n, nlev) - a0;
+    constexpr int LD = DC_SMALL_LD;
+    // the leaves' eigenpairs (1x1 and 2x2 blocks on the diagonal)
+    for (int e = tid; e < S * S; e += DC_SMALL_NT) {
+        const int r = e / S, c = e - r * S;
+        sm.V[r * LD + c] = 0.0;
+    }
+    __syncthreads();
+    for (int l = tid; l < (1 << ls); l += DC_SMALL_NT) {
+        const int la = dc_bound((k0 << ls) + l, n, nlev), lb = dc_bound((k0 << ls) + l + 1, n, nlev);
+        for (int r = la; r < lb; ++r)
+            for (int c = la; c < lb; ++c) sm.V[(r - a0) * LD + (c - a0)] = Vg[(size_t)r * n + c];
+    }
+    for (int i = tid; i < S; i += DC_SMALL_NT) {
+        sm.D[i] = Dg[a0 + i];
+        sm.BETA[i] = betag[a0 + i];
+    }
+    __syncthreads();
+    for (int L = 1; L <= ls; ++L) {
+        const int nm = 1 << (ls - L);
+        int a, mid, b;
+        // deflation, then the gather (lane = column) by the same wavefront,
+        // one wavefront per merge (LDS operations of a wavefront are in order)
+        for (int m = wave; m < nm; m += DC_SMALL_NW) {
+            dc_small_bounds(n, nlev, ls, k0, a0, L, m, a, mid, b);
+            dc_small_deflate(sm, m, a, mid, b, lane);
+            const int K = b - a;
+            if (lane < K)
+                dc_gather_seg(sm.V + a * LD + a, sm.G + a * LD + a, LD, mid - a, 0, K, lane, sm.PERM + a,
+                              sm.POS + a, sm.KIND + a, sm.RC + a, sm.RS + a);
+        }
+        __syncthreads();
+        // root slot = row of 16 lanes (64 slots, and at most 64 roots per
+        // level): slot t takes root t of the level, merges in order
+        const int slot = tid >> 4, sl = tid & 15;
+        int mm = -1, jr = 0;
+        {
+            int base = 0;
+            for (int m = 0; m < nm; ++m) {
+                const int kp = sm.KP[m];
+                if (slot >= base && slot < base + kp) {
+                    mm = m;
+                    jr = slot - base;
+                }
+                base += kp;
+            }
+        }
+        const int ra = mm >= 0 ? sm.MA[mm] : 0, rkp = mm >= 0 ? sm.KP[mm] : 0;
+        const double *rdd = sm.DD + ra;
+        // secular root jr, and row jr of Delta
+        if (mm >= 0) {
+            int org;
+            double tau;
+            secular_root<16>(rdd, sm.ZZ + ra, rkp, sm.RHO[mm], jr, sl, org, tau);
+            const double dorg = rdd[org];
+            if (sl == 0) sm.D[ra + jr] = dorg + tau;
+            double *row = sm.U + (ra + jr) * LD + ra;
+            for (int i = sl; i < rkp; i += 16) row[i] = (rdd[i] - dorg) - tau;
+        }
+        __syncthreads();
+        // z-hat_i^2 = -Delta_ii prod_{j != i} Delta_ji / (dd_i - dd_j) / rho, i = jr
+        if (mm >= 0) {
+            const double *Dl = sm.U + ra * LD + ra;
+            const double di = rdd[jr];
+            double w = 1.0;
+            for (int j = sl; j < rkp; j += 16)
+                w *= j == jr ? Dl[jr * LD + jr] : Dl[j * LD + jr] / (di - rdd[j]);
+            w = row_prod(w);
+            if (sl == 0) sm.ZH[ra + jr] = copysign(sqrt(fmax(-w / sm.RHO[mm], 0.0)), sm.ZZ[ra + jr]);
+        }
+        __syncthreads();
+        // row jr of U = z-hat / Delta, normalised
+        if (mm >= 0) {
+            const double *zh = sm.ZH + ra;
+            double *row = sm.U + (ra + jr) * LD + ra;
+            double x[4], ss = 0.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = sl + 16 * q;
+                x[q] = i < rkp ? zh[i] / row[i] : 0.0;
+                ss += x[q] * x[q];
+            }
+            const double inv = 1.0 / sqrt(row_sum(ss));
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = sl + 16 * q;
+                if (i < rkp) row[i] = x[q] * inv;
+            }
+        }
+        __syncthreads();
+        // merge product: row r of merge [a, b) <- (U G)_r for r - a < K',
+        // the gathered row (a deflated pole) otherwise; 16 threads per row
+        {
+            const int r = tid >> 4, c0 = tid & 15;
+            if (r < S) {
+                const int m = sm.ROWM[r];
+                a = sm.MA[m];
+                const int K = sm.MB[m] - a, kp = sm.KP[m], j = r - a;
+                int col[4];  // columns of this thread, clamped into the merge
+#pragma unroll
+                for (int q = 0; q < 4; ++q) col[q] = a + min(c0 + 16 * q, K - 1);
+                double acc[4] = {0.0, 0.0, 0.0, 0.0};
+                if (j < kp) {
+                    const double *u = sm.U + r * LD + a;
+                    const double *g = sm.G + a * LD;
+                    for (int i = 0; i < kp; ++i) {
+                        const double ui = u[i];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) acc[q] += ui * g[i * LD + col[q]];
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[q] = sm.G[r * LD + col[q]];
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (c0 + 16 * q < K) sm.V[r * LD + col[q]] = acc[q];
+            }
+        }
+        __syncthreads();
+    }
+    // eigenvalues and the S x S eigenvector block of the subtree
+    for (int e = tid; e < S * S; e += DC_SMALL_NT) {
+        const int r = e / S, c = e - r * S;
+        Vg[(size_t)(a0 + r) * n + a0 + c] = sm.V[r * LD + c];
+    }
+    for (int i = tid; i < S; i += DC_SMALL_NT) Dg[a0 + i] = sm.D[i];
 }
 
 // Stable ascending rank of the final eigenvalues; W[rank] = D.
@@ -1539,7 +1837,13 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         double flops;
     };
     std::vector<Level> levels(maxlev + 1);
-    for (int L = 1; L <= maxlev; ++L) {
+    // levels 1..DC_SMALL_LEV: one k_dc_small workgroup per subtree
+    std::vector<BtItem> small;
+    for (int i = 0; i < nbig; ++i) {
+        const int nlev = dc_levels(tasks[i].n), ls = std::min(DC_SMALL_LEV, nlev);
+        for (int k = 0; k < (1 << (nlev - ls)); ++k) small.push_back({i, k});
+    }
+    for (int L = DC_SMALL_LEV + 1; L <= maxlev; ++L) {
         Level lv{};
         lv.m0 = (int)merges.size();
         lv.s0 = (int)schunks.size();
@@ -1697,6 +2001,8 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         c->tri_capacity = std::max(1, per_cu * ncu);
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_reg, REG_NT, 0));
         c->reg_capacity = std::max(1, per_cu * ncu);
+        HIPCHK(c, hipFuncSetAttribute((const void *)k_dc_small, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)sizeof(DcSmall)));
     }
     // Team sizes, then longest-processing-time assignment of the jobs to the
     // XCD queues (work = estimated time x workgroups; a team stays on one XCD)
@@ -1793,6 +2099,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     HostArena ar;
     const size_t o_jobs = ar.put(jobs), o_tb = ar.put(tbase), o_qj = ar.put(qjob), o_ones = ar.put(ones);
     const size_t o_rjobs = ar.put(rjobs), o_rtb = ar.put(rtbase), o_rqj = ar.put(rqjob);
+    const size_t o_small = ar.put(small);
     const size_t o_mats = ar.put(mats), o_merges = ar.put(merges), o_sch = ar.put(schunks), o_uch = ar.put(uchunks),
                  o_xch = ar.put(gathers), o_cpy = ar.put(copies), o_gouts = ar.put(gouts), o_gterms = ar.put(gterms), o_gitems = ar.put(gitems), o_gred = ar.put(gred), o_rk = ar.put(rk),
                  o_cp = ar.put(cp);
@@ -1871,6 +2178,12 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     {
         LaunchTimer lt(K_DC_LEAF, c->stream, 0.0, 0.0);
         hipLaunchKernelGGL(k_dc_leaf, dim3(nbig), dim3(256), 0, c->stream, d_mats);
+    }
+    HIPCHK(c, hipGetLastError());
+    {
+        LaunchTimer lt(K_DC_SMALL, c->stream, 0.0, 0.0);
+        hipLaunchKernelGGL(k_dc_small, dim3(small.size()), dim3(DC_SMALL_NT), sizeof(DcSmall), c->stream, d_mats,
+                           (const BtItem *)(db + o_small));
     }
     HIPCHK(c, hipGetLastError());
     for (int L = 1; L <= maxlev; ++L) {

@@ -73,6 +73,7 @@ enum KernelId {
     K_EIG_TRI1,
     K_EIG_TRI_REG,
     K_DC_LEAF,
+    K_DC_SMALL,
     K_DC_DEFLATE,
     K_DC_SECULAR,
     K_DC_ZHAT,
@@ -152,11 +153,24 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
 // mirror, row mirror), then the four row sums read as scalars and added in a
 // fixed order, so every lane returns the same bits.  No LDS traffic, unlike
 // shuffles, which all go through the CU's LDS unit.
-__device__ inline double wave_sum(double v) {
+// Sum over each row of 16 lanes (every lane of the row active), the same
+// bits in all 16 lanes: the row-local DPP steps of wave_sum.
+__device__ inline double row_sum(double v) {
     v += dpp_d<0xB1>(v);
     v += dpp_d<0x4E>(v);
     v += dpp_d<0x141>(v);
     v += dpp_d<0x140>(v);
+    return v;
+}
+__device__ inline double row_prod(double v) {
+    v *= dpp_d<0xB1>(v);
+    v *= dpp_d<0x4E>(v);
+    v *= dpp_d<0x141>(v);
+    v *= dpp_d<0x140>(v);
+    return v;
+}
+__device__ inline double wave_sum(double v) {
+    v = row_sum(v);
     return (readlane_d(v, 0) + readlane_d(v, 16)) + (readlane_d(v, 32) + readlane_d(v, 48));
 }
 __device__ inline double wave_max(double v) {
