@@ -1274,25 +1274,35 @@ __global__ __launch_bounds__(256) void k_dc_secular(const DcMerge *merges, const
     for (int i = lane; i < kp; i += 64) row[i] = (dd[i] - dorg) - tau;
 }
 
-// z-hat_i^2 = -Delta_ii prod_{j != i} Delta_ji / (dd_i - dd_j) / rho
-__global__ __launch_bounds__(SEC_NT) void k_dc_zhat(const DcMerge *merges, const DcChunk *chunks) {
+// z-hat_i^2 = -Delta_ii prod_{j != i} Delta_ji / (dd_i - dd_j) / rho: lane l
+// takes i = j0 + l (loads of a row of Delta coalesced over i), wavefront w
+// the factors j = w mod ZHAT_NW; the partial products meet in LDS.
+constexpr int ZHAT_NW = 16;
+__global__ __launch_bounds__(64 * ZHAT_NW) void k_dc_zhat(const DcMerge *merges, const DcChunk *chunks) {
+    __shared__ double part[ZHAT_NW][64];
     const DcChunk ck = chunks[blockIdx.x];
     const DcMerge mg = merges[ck.merge];
     const int n = mg.n, a = mg.a;
     double *ws = mg.ws;
     const int kp = islot(ws, n, V_META)[a];
-    const int i = ck.j0 + threadIdx.x;
-    if (i >= kp) return;
-    const double rho = vslot(ws, n, V_RHO)[a];
-    const double *dd = vslot(ws, n, V_DD) + a, *zz = vslot(ws, n, V_ZZ) + a;
+    if (ck.j0 >= kp) return;  // the whole workgroup
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = ck.j0 + lane;
+    const double *dd = vslot(ws, n, V_DD) + a;
     const double *S2 = ws + 2 * (size_t)n * n + (size_t)a * n + a;
-    const double di = dd[i];
-    double w = S2[(size_t)i * n + i];
-    for (int j = 0; j < kp; ++j) {
-        if (j == i) continue;
-        w *= S2[(size_t)j * n + i] / (di - dd[j]);
+    double w = 1.0;
+    if (i < kp) {
+        const double di = dd[i];
+        for (int j = wave; j < kp; j += ZHAT_NW) w *= j == i ? S2[(size_t)i * n + i] : S2[(size_t)j * n + i] / (di - dd[j]);
     }
-    vslot(ws, n, V_ZH)[a + i] = copysign(sqrt(fmax(-w / rho, 0.0)), zz[i]);
+    part[wave][lane] = w;
+    __syncthreads();
+    if (wave == 0 && i < kp) {
+        double p = part[0][lane];
+        for (int g = 1; g < ZHAT_NW; ++g) p *= part[g][lane];
+        const double rho = vslot(ws, n, V_RHO)[a];
+        vslot(ws, n, V_ZH)[a + i] = copysign(sqrt(fmax(-p / rho, 0.0)), vslot(ws, n, V_ZZ)[a + i]);
+    }
 }
 
 // Row j of S2 <- z-hat / Delta_j, normalised; one wavefront per row.
@@ -2215,7 +2225,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         }
         {
             LaunchTimer lt(K_DC_ZHAT, c->stream, 0.0, 0.0);
-            hipLaunchKernelGGL(k_dc_zhat, dim3(lv.s1 - lv.s0), dim3(SEC_NT), 0, c->stream, d_merges, d_sch + lv.s0);
+            hipLaunchKernelGGL(k_dc_zhat, dim3(lv.s1 - lv.s0), dim3(64 * ZHAT_NW), 0, c->stream, d_merges, d_sch + lv.s0);
         }
         {
             LaunchTimer lt(K_DC_UNORM, c->stream, 0.0, 0.0);
