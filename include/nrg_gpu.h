@@ -131,7 +131,10 @@ nrg_status nrg_assemble_batch(nrg_ctx *ctx, const nrg_asm_block *blocks, int nbl
  * Eigen::matrix0 / Eigen::value, src/nrg.cc:285-376).  Workspace is internal
  * (3 n^2 + O(n) doubles per matrix).  n is limited to 32768; larger matrices
  * return NRG_ERR_ARG before any device access.  NRG_ERR_NOCONV is reserved (the divide-and-conquer
- * solver has no failing iteration). */
+ * solver has no failing iteration).  Queued like the other batched calls: a
+ * tridiagonalisation team that does not assemble within its bounded wait is
+ * reported (NRG_ERR_HIP) by the next call that waits for the stream
+ * (nrg_sync, the transfers to the host). */
 typedef struct {
     double *A;
     double *W;

@@ -54,7 +54,13 @@ nrg_status wait_event(nrg_ctx *c, hipEvent_t e) {
 
 nrg_status wait_stream(nrg_ctx *c) {
     HIPCHK(c, hipEventRecord(c->ev_wait, c->stream));
-    return wait_event(c, c->ev_wait);
+    nrg_status s = wait_event(c, c->ev_wait);
+    if (s) return s;
+    if (c->status_pending) {
+        c->status_pending = false;
+        if (*c->hstat) return fail(c, NRG_ERR_HIP, "diag: a tridiagonalisation team did not assemble in time");
+    }
+    return NRG_OK;
 }
 
 nrg_status stage(nrg_ctx *c, const void *host, size_t bytes, void **dev) {

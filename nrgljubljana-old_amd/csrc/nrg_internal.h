@@ -47,6 +47,9 @@ struct nrg_ctx {
     // pinned host memory: gathers (written by kernels directly), the
     // eigensolver status
     int *hstat = nullptr;
+    // a queued eigensolve's status word has not been read yet (wait_stream
+    // reads it once the stream has drained)
+    bool status_pending = false;
     double *hpin = nullptr;
     size_t hpin_doubles = 0;
 };
@@ -124,7 +127,8 @@ struct LaunchTimer {
 
 // Waits for an event / for everything queued on the context's stream by
 // polling (the runtime's blocking waits may sleep well past completion,
-// which leaves the GPU idle until the host wakes up).
+// which leaves the GPU idle until the host wakes up).  wait_stream also
+// reports the status of eigensolves queued since the last wait.
 nrg_status wait_event(nrg_ctx *c, hipEvent_t e);
 nrg_status wait_stream(nrg_ctx *c);
 

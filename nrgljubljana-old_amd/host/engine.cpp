@@ -418,47 +418,35 @@ struct Engine {
     }
 
     // ------------------------------------------------------------ truncation
-    // find_clusters + fix_splittings (src/splitting.cc:5-77)
+    // find_clusters + fix_splittings (src/splitting.cc:5-77): in ascending
+    // order, a cluster starts at the first value at least fixeps above the
+    // current cluster's first value, and every member takes that first value.
+    // One sort of (value, location); the fixed values, still ascending, are
+    // kept in it.sorted for the truncation.
     void fix_splittings(Iter &it) {
-        std::vector<double> e;
-        for (auto &kv : it.subs) e.insert(e.end(), kv.second.value.begin(), kv.second.value.end());
-        std::sort(e.begin(), e.end());
-        std::map<double, double> mapping;
-        double e0 = e[0];
-        size_t i0 = 0;
-        int size = 1;
-        for (size_t i = 0; i <= e.size(); ++i) {
-            if (i < e.size() && (e[i] - e0) < P.fixeps) {
-                size++;
-                continue;
-            }
-            if (size > 1) {
-                bool split = false;
-                for (size_t a = i0; a < i; ++a)
-                    if (e[a] != e[i0]) split = true;
-                if (split)
-                    for (size_t a = i0 + 1; a < i; ++a)
-                        if (e[a] != e[i0]) mapping.emplace(e[a], e[i0]);
-            }
-            if (i < e.size()) {
-                e0 = e[i];
-                i0 = i;
-                size = 1;
-            }
-        }
-        if (mapping.empty()) return;
+        std::vector<std::pair<double, double *>> e;
         for (auto &kv : it.subs)
-            for (auto &x : kv.second.value) {
-                auto m = mapping.find(x);
-                if (m != mapping.end()) x = m->second;
-            }
+            for (double &x : kv.second.value) e.push_back({x, &x});
+        std::sort(e.begin(), e.end(), [](const std::pair<double, double *> &a, const std::pair<double, double *> &b) {
+            return a.first < b.first;
+        });
+        it.sorted.resize(e.size());
+        size_t i0 = 0;
+        for (size_t i = 0; i < e.size(); ++i) {
+            if (!(e[i].first - e[i0].first < P.fixeps)) i0 = i;
+            const double rep = e[i0].first;
+            *e[i].second = rep;
+            it.sorted[i] = rep;
+        }
     }
 
     // highest_retained_energy + nrg_truncate_prepare (src/nrg.cc:1919-1998)
     void truncate_prepare(Iter &it, bool last, bool dmnrgrun) {
-        std::vector<double> e;
-        for (auto &kv : it.subs) e.insert(e.end(), kv.second.value.begin(), kv.second.value.end());
-        std::sort(e.begin(), e.end());
+        if (it.sorted.empty()) {
+            for (auto &kv : it.subs) it.sorted.insert(it.sorted.end(), kv.second.value.begin(), kv.second.value.end());
+            std::sort(it.sorted.begin(), it.sorted.end());
+        }
+        const std::vector<double> &e = it.sorted;
         const size_t total = e.size();
         size_t nrkeep;
         if (P.keepenergy <= 0.0) nrkeep = P.keep;

@@ -154,6 +154,7 @@ struct Iter {
     int N = 0;
     double scale = 0.0, Egs = 0.0;
     std::map<Invar, Sub> subs;
+    std::vector<double> sorted;    // all energies (shifted, splittings fixed), ascending
     std::unique_ptr<Arena> arena;  // U, W (one chunk)
     // spill to pinned host memory when the eigenvectors of all iterations do
     // not fit in the device budget (the reference's workdir/unitaryN files)
