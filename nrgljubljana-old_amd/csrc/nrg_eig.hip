@@ -417,7 +417,7 @@ __device__ __forceinline__ void wave_sum_n(double (&v)[R]) {
 
 template <int NC, int RPW, int NW, bool TEAM>
 __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *bar, int *status, int *s_flag,
-                                        double *lbuf) {
+                                        double *lbuf, double *vsh) {
     const int n = mt.n;
     double *__restrict__ A = mt.A;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -506,11 +506,18 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
                 xr[i] = ok ? rl : 0.0;
                 pp += vv[i] * w[i];
             }
-            // p at column j+1 and at the own rows, beside the v.p reduction
-            const double p1 = reg_at<NC>(w, j + 1);
+            // p at column j+1 and at the own rows (read in LDS), and v_j at
+            // the own rows (written to LDS by wavefront 0 at the end of the
+            // previous step)
+            const double p1 = pg[j + 1];
             double prow[RPW];
 #pragma unroll
-            for (int s = 0; s < RPW; ++s) prow[s] = reg_at<NC>(w, row[s]);
+            for (int s = 0; s < RPW; ++s) {
+                const int r = row[s];
+                const double x = pg[min(r, n - 1)];
+                prow[s] = (r > j && r < n) ? x : 0.0;
+                vrow[s] = vsh[(j & 1) * 64 * NC + min(r, 64 * NC - 1)];
+            }
             const double K = 0.5 * tj * wave_sum(pp);
 #pragma unroll
             for (int i = 0; i < NC; ++i) w[i] -= K * vv[i];
@@ -591,13 +598,15 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
                 }
             }
         }
-        // v_{j+1} becomes current; its entries at the own rows are gathered
-        // while the stores drain
+        // v_{j+1} becomes current; wavefront 0 leaves it in LDS for the
+        // entries at the own rows (read after the step's barrier)
 #pragma unroll
         for (int i = 0; i < NC; ++i) vv[i] = nv[i];
         v1 = live ? v0 : 0.0;
+        if (wave == 0) {
 #pragma unroll
-        for (int s = 0; s < RPW; ++s) vrow[s] = reg_at<NC>(nv, row[s]);
+            for (int i = 0; i < NC; ++i) vsh[((j + 1) & 1) * 64 * NC + lane + 64 * i] = nv[i];
+        }
         tj = tn;
         if (TEAM) {
             ++phase;
@@ -622,8 +631,9 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
 // Matrices up to REG_ONE_N: one workgroup each.
 __global__ __launch_bounds__(TRI_NT) void k_eig_tri_one(const EigMat *mats, const int *idx, int *status) {
     __shared__ double lbuf[4 * REG_ONE_N];
+    __shared__ double vsh[2 * 64 * 2];
     const EigMat mt = mats[idx[blockIdx.x]];
-    tri_reg<2, 8, TRI_NW, false>(mt, 0, 1, nullptr, status, nullptr, lbuf);
+    tri_reg<2, 8, TRI_NW, false>(mt, 0, 1, nullptr, status, nullptr, lbuf, vsh);
 }
 
 // One persistent launch for the register-resident teams: workgroups of XCD
@@ -632,6 +642,7 @@ __global__ __launch_bounds__(REG_NT) void k_eig_tri_reg(const EigMat *mats, cons
                                                         const int *qjob, int *counters, int *status) {
     __shared__ int s_ticket, s_flag;
     __shared__ double lbuf[2 * REG_MAX_N];
+    __shared__ double vsh[2 * 64 * (REG_MAX_N / 64)];
     const int x = blockIdx.x % N_XCD;
     int *ticket = counters + x;
     int *bars = counters + N_XCD + 1;
@@ -652,9 +663,9 @@ __global__ __launch_bounds__(REG_NT) void k_eig_tri_reg(const EigMat *mats, cons
         const TriJob jb = jobs[lo];
         const EigMat mt = mats[jb.mat];
         const int rank = t0 + t - tbase[lo];
-        if (mt.n <= 256) tri_reg<4, 8, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf);
-        else if (mt.n <= 384) tri_reg<6, 4, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf);
-        else tri_reg<9, 3, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf);
+        if (mt.n <= 256) tri_reg<4, 8, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf, vsh);
+        else if (mt.n <= 384) tri_reg<6, 4, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf, vsh);
+        else tri_reg<9, 3, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf, vsh);
         __syncthreads();
     }
 }
