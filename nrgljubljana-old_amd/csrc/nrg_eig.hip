@@ -372,33 +372,19 @@ __device__ __forceinline__ void tri_team(const EigMat &mt, int rank, int T, int 
 //     register budget (256 per lane) holds RPW x NC doubles of rows plus the
 //     step's vectors without spilling.  Per step j every
 //     wavefront redundantly rebuilds w_j, the updated row j+1 and v_{j+1}
-//     for its own columns from p_j and row j+1 as published by step j-1, so
-//     the only block-level synchronisation of a step is the exchange itself
-//     (one team barrier).  Reductions are wave butterflies, which leave the
-//     same bits in every lane, so all copies of v_{j+1} and tau_{j+1} agree.
+//     for its own columns from p_j and row j+1 as published by step j-1
+//     (read once per workgroup into LDS, beside v_j, which wavefront 0 leaves
+//     there), so a step synchronises only at the exchange (one team barrier)
+//     and at that staging.  The own-row update is issued before the
+//     reflector's reductions and square root, whose latency it fills.
+//     Reductions are wave butterflies, which leave the same bits in every
+//     lane, so all copies of v_{j+1} and tau_{j+1} agree.
 //     Output as tri_core: reflector j in row j of A (columns j+1..), tau,
 //     diagonal and off-diagonal.
 // ---------------------------------------------------------------------------
 constexpr int REG_ONE_N = 128;  // one workgroup: 16 waves x 8 rows, 2 column registers
 constexpr int REG_MAX_N = 576;  // largest team variant: 9 column registers
 constexpr int REG_NT = 512;     // team workgroups: 8 waves
-
-// x[col >> 6] of lane col & 63 (col the same in every lane), as a scalar: a
-// chain of uniform branches, each reading one fixed register (a select over
-// the registers would be turned into an indexed array in memory)
-template <int NC, int K = 0>
-__device__ __forceinline__ double reg_pick(const double (&x)[NC], int i, int lane) {
-    if constexpr (K == NC - 1) {
-        return readlane_d(x[K], lane);
-    } else {
-        if (i == K) return readlane_d(x[K], lane);
-        return reg_pick<NC, K + 1>(x, i, lane);
-    }
-}
-template <int NC>
-__device__ __forceinline__ double reg_at(const double (&x)[NC], int col) {
-    return reg_pick<NC>(x, __builtin_amdgcn_readfirstlane(col >> 6), __builtin_amdgcn_readfirstlane(col & 63));
-}
 
 // team size of the register variant for dimension n (REG_ONE_N < n <=
 // REG_MAX_N): workgroups of 8 waves x (8, 4, 3) rows
@@ -437,6 +423,10 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
     double *const pb0 = TEAM ? vslot(ws, n, V_PV) : lbuf, *const pb1 = TEAM ? vslot(ws, n, V_DGT) : lbuf + n;
     double *const rb0 = TEAM ? vslot(ws, n, V_ODT) : lbuf + 2 * n, *const rb1 = TEAM ? vslot(ws, n, V_VB) : lbuf + 3 * n;
     const bool lead = rank == 0 && wave == 0;
+    if constexpr (TEAM) {
+        // staging buffer: columns outside the active range read as zero
+        for (int k = tid; k < 128 * NC; k += 64 * NW) lbuf[k] = 0.0;  // first read after the step's barrier
+    }
     int row[RPW];
 #pragma unroll
     for (int s = 0; s < RPW; ++s) row[s] = rank + T * (wave + NW * s);
@@ -461,7 +451,7 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
     double tj = 0.0;
     int phase = 0;
     for (int j = -1; j <= n - 3; ++j) {
-        double w[NC], xr[NC], w1 = 0.0, wrow[RPW];
+        double w[NC], xr[NC], w1 = 0.0, wrow[RPW], x0, x1;
         if (j < 0) {
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
@@ -472,6 +462,8 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
             }
 #pragma unroll
             for (int s = 0; s < RPW; ++s) wrow[s] = 0.0;
+            x0 = A[0];
+            x1 = A[1];
         } else {
             const double *pg = ((j + 2) & 1) ? pb1 : pb0, *rg = ((j + 2) & 1) ? rb1 : rb0;
             if constexpr (TEAM) {
@@ -485,25 +477,36 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
                     const int k = min(tid + 64 * NW * u, 2 * m - 1);
                     t[u] = k < m ? ld_dev(pg + j + 1 + k) : ld_dev(rg + j + 1 + k - m);
                 }
+                // (p in lbuf[0, 64 NC), the row in lbuf[64 NC, 128 NC): zero
+                // outside columns j+1..n-1, column j cleared here)
 #pragma unroll
                 for (int u = 0; u < NL; ++u) {
                     const int k = tid + 64 * NW * u;
-                    if (k < 2 * m) lbuf[k < m ? j + 1 + k : n + j + 1 + k - m] = t[u];
+                    if (k < 2 * m) lbuf[k < m ? j + 1 + k : 64 * NC + j + 1 + k - m] = t[u];
+                }
+                if (tid == 0) {
+                    lbuf[j] = 0.0;
+                    lbuf[64 * NC + j] = 0.0;
                 }
                 __syncthreads();
                 pg = lbuf;
-                rg = lbuf + n;
+                rg = lbuf + 64 * NC;
             }
             double pp = 0.0;
 #pragma unroll
             for (int i = 0; i < NC; ++i) {
                 const int c = lane + 64 * i;
-                const bool ok = c > j && c < n;
-                const unsigned cc = (unsigned)min(c, n - 1);
-                const double pl = pg[cc];
-                const double rl = rg[cc];
-                w[i] = ok ? pl : 0.0;
-                xr[i] = ok ? rl : 0.0;
+                if constexpr (TEAM) {
+                    w[i] = pg[c];
+                    xr[i] = rg[c];
+                } else {
+                    const bool ok = c > j && c < n;
+                    const unsigned cc = (unsigned)min(c, n - 1);
+                    const double pl = pg[cc];
+                    const double rl = rg[cc];
+                    w[i] = ok ? pl : 0.0;
+                    xr[i] = ok ? rl : 0.0;
+                }
                 pp += vv[i] * w[i];
             }
             // p at column j+1 and at the own rows (read in LDS), and v_j at
@@ -526,55 +529,53 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
             for (int s = 0; s < RPW; ++s) wrow[s] = prow[s] - K * vrow[s];
 #pragma unroll
             for (int i = 0; i < NC; ++i) xr[i] -= v1 * w[i] + w1 * vv[i];
-            if (lead) {
+            // row j+1 at columns j+1 and j+2 from the LDS copies (the same
+            // inputs in every wavefront; these two entries are used only for
+            // the diagonal and the next reflector's pivot)
+            const double vv2 = vsh[(j & 1) * 64 * NC + j + 2];
+            const double w2 = pg[j + 2] - K * vv2;
+            x0 = rg[j + 1] - (v1 * w1 + w1 * v1);
+            x1 = rg[j + 2] - (v1 * w2 + w1 * vv2);
+        }
+        // own rows: update with (v_j, w_j), independent of the reflector
+        // below, so that it fills the latency of that chain (finished or
+        // padding rows are updated too, but never published)
 #pragma unroll
-                for (int i = 0; i < NC; ++i) {
-                    const int c = lane + 64 * i;
-                    if (c > j && c < n) A[(size_t)j * n + c] = vv[i];  // reflector j
-                }
-            }
+        for (int s = 0; s < RPW; ++s) {
+#pragma unroll
+            for (int i = 0; i < NC; ++i) a[s][i] -= vrow[s] * w[i] + wrow[s] * vv[i];
         }
         // row j+1 is final: diagonal x0, and the reflector that zeroes it
-        // beyond column j+2
-        const double x0 = reg_at<NC>(xr, j + 1), x1 = reg_at<NC>(xr, j + 2);
+        // beyond column j+2 (branch-free: a uniform choice of results)
         double ps = 0.0;
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
             const int c = lane + 64 * i;
-            if (c > j + 2 && c < n) ps += xr[i] * xr[i];
+            if (c > j + 2 && (TEAM || c < n)) ps += xr[i] * xr[i];  // (team rows are zero past n)
         }
         const double sigma = wave_sum(ps);
         const bool has_next = j + 1 <= n - 3;
-        double tn = 0.0, alpha = x1, v0 = 0.0;
-        if (has_next && sigma != 0.0) {
-            alpha = -copysign(sqrt(x1 * x1 + sigma), x1);
-            v0 = x1 - alpha;
-            tn = 2.0 / (v0 * v0 + sigma);
-        }
+        const bool refl = has_next && sigma != 0.0;
+        const double sg = refl ? sigma : 1.0;
+        const double at = -copysign(sqrt(x1 * x1 + sg), x1);
+        const double alpha = refl ? at : x1;
+        const double v0 = refl ? x1 - at : 0.0;
+        const double tn = refl ? 2.0 / (v0 * v0 + sg) : 0.0;
         const bool live = tn != 0.0;
         double nv[NC];
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
             const int c = lane + 64 * i;
-            nv[i] = !live ? 0.0 : (c == j + 2) ? v0 : ((c > j + 2 && c < n) ? xr[i] : 0.0);
+            nv[i] = !live ? 0.0 : (c == j + 2) ? v0 : ((c > j + 2 && (TEAM || c < n)) ? xr[i] : 0.0);
         }
-        if (lead && lane == 0) {
-            gdg[j + 1] = x0;
-            god[j + 1] = alpha;
-            if (has_next) tau[j + 1] = tn;
-        }
-        // own rows: update with (v_j, w_j) and p_{j+1}[r] = tn row_r . v_{j+1},
-        // computed for every slot (finished or padding rows are not
-        // published); the owner of row j+2 publishes that row
+        // p_{j+1}[r] = tn row_r . v_{j+1}; the owner of row j+2 publishes
+        // that row
         double acc[RPW];
 #pragma unroll
         for (int s = 0; s < RPW; ++s) {
             acc[s] = 0.0;
 #pragma unroll
-            for (int i = 0; i < NC; ++i) {
-                a[s][i] -= vrow[s] * w[i] + wrow[s] * vv[i];
-                acc[s] += a[s][i] * nv[i];
-            }
+            for (int i = 0; i < NC; ++i) acc[s] += a[s][i] * nv[i];
         }
         wave_sum_n<RPW>(acc);
         double *pn = ((j + 3) & 1) ? pb1 : pb0, *rn = ((j + 3) & 1) ? rb1 : rb0;
@@ -596,6 +597,22 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
                         }
                     }
                 }
+            }
+        }
+        // the lead wavefront's outputs, off the critical path: reflector j,
+        // diagonal, off-diagonal and tau
+        if (lead) {
+            if (j >= 0) {
+#pragma unroll
+                for (int i = 0; i < NC; ++i) {
+                    const int c = lane + 64 * i;
+                    if (c > j && c < n) A[(size_t)j * n + c] = vv[i];
+                }
+            }
+            if (lane == 0) {
+                gdg[j + 1] = x0;
+                god[j + 1] = alpha;
+                if (has_next) tau[j + 1] = tn;
             }
         }
         // v_{j+1} becomes current; wavefront 0 leaves it in LDS for the
@@ -641,7 +658,7 @@ __global__ __launch_bounds__(TRI_NT) void k_eig_tri_one(const EigMat *mats, cons
 __global__ __launch_bounds__(REG_NT) void k_eig_tri_reg(const EigMat *mats, const TriJob *jobs, const int *tbase,
                                                         const int *qjob, int *counters, int *status) {
     __shared__ int s_ticket, s_flag;
-    __shared__ double lbuf[2 * REG_MAX_N];
+    __shared__ double lbuf[2 * 64 * (REG_MAX_N / 64)];
     __shared__ double vsh[2 * 64 * (REG_MAX_N / 64)];
     const int x = blockIdx.x % N_XCD;
     int *ticket = counters + x;
