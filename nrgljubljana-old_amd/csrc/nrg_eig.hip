@@ -1641,22 +1641,32 @@ __global__ __launch_bounds__(DC_SMALL_NT) void k_dc_small(const EigMat *mats, co
     for (int i = tid; i < S; i += DC_SMALL_NT) Dg[a0 + i] = sm.D[i];
 }
 
-// Stable ascending rank of the final eigenvalues; W[rank] = D.
-__global__ __launch_bounds__(256) void k_dc_rank(const EigMat *mats, const BtItem *items) {
+// Stable ascending rank of the final eigenvalues; W[rank] = D.  Lane l takes
+// eigenvalue c0 + l, wavefront w counts over j = w mod RANK_NW; the partial
+// counts meet in LDS.
+constexpr int RANK_NW = 16;
+__global__ __launch_bounds__(64 * RANK_NW) void k_dc_rank(const EigMat *mats, const BtItem *items) {
+    __shared__ int part[RANK_NW][64];
     const BtItem it = items[blockIdx.x];
     const EigMat mt = mats[it.mat];
     const int n = mt.n;
-    const int i = it.c0 + threadIdx.x;
-    if (i >= n) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = it.c0 + lane;
     const double *D = vslot(mt.ws, n, V_D);
-    const double di = D[i];
+    const double di = D[min(i, n - 1)];
     int rk = 0;
-    for (int j = 0; j < n; ++j) {
+    for (int j = wave; j < n; j += RANK_NW) {
         const double dj = D[j];
         rk += (dj < di) || (dj == di && j < i);
     }
-    islot(mt.ws, n, V_RANK)[i] = rk;
-    mt.W[rk] = di;
+    part[wave][lane] = rk;
+    __syncthreads();
+    if (wave == 0 && i < n) {
+        int r = 0;
+        for (int g = 0; g < RANK_NW; ++g) r += part[g][lane];
+        islot(mt.ws, n, V_RANK)[i] = r;
+        mt.W[r] = di;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1961,7 +1971,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             const size_t nn = (size_t)n * n;
             for (size_t e = 0; e < nn; e += 256 * 16) cp.push_back({i, (int)(e / (256 * 16))});
         }
-        for (int c0 = 0; c0 < n; c0 += 256) rk.push_back({i, c0});
+        for (int c0 = 0; c0 < n; c0 += 64) rk.push_back({i, c0});
     }
     // WY blocks, counted from the last: block q of matrix i covers reflectors
     // [max(0, j1 - WY_NB), j1) with j1 = n - 2 - q WY_NB
@@ -2269,7 +2279,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     // 3. sort, back-transformation, copy into A
     {
         LaunchTimer lt(K_DC_RANK, c->stream, 0.0, 0.0);
-        hipLaunchKernelGGL(k_dc_rank, dim3(rk.size()), dim3(256), 0, c->stream, d_mats, (const BtItem *)(db + o_rk));
+        hipLaunchKernelGGL(k_dc_rank, dim3(rk.size()), dim3(64 * RANK_NW), 0, c->stream, d_mats, (const BtItem *)(db + o_rk));
     }
     HIPCHK(c, hipGetLastError());
     {
@@ -2281,9 +2291,26 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             by += 8.0 * 2.0 * n * n;
         }
         LaunchTimer lt(K_EIG_BT, c->stream, fl, by);
+        // the variants work on different matrices: the largest runs on the
+        // stream, the others beside it on the side stream
+        int vmax = -1;
         for (int v = 0; v < kBtVariants; ++v)
-            if (!bt[v].empty())
-                launch_bt_variant(v, c->stream, (int)bt[v].size(), d_mats, (const BtItem *)(db + o_bt[v]));
+            if (!bt[v].empty()) vmax = v;
+        bool forked = false;
+        for (int v = 0; v < kBtVariants; ++v) {
+            if (bt[v].empty() || v == vmax) continue;
+            if (!forked) {
+                HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+                HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+                forked = true;
+            }
+            launch_bt_variant(v, c->side, (int)bt[v].size(), d_mats, (const BtItem *)(db + o_bt[v]));
+        }
+        if (vmax >= 0) launch_bt_variant(vmax, c->stream, (int)bt[vmax].size(), d_mats, (const BtItem *)(db + o_bt[vmax]));
+        if (forked) {
+            HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+            HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        }
     }
     HIPCHK(c, hipGetLastError());
     if (!cp.empty()) {

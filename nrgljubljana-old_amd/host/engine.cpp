@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <fstream>
 #include <iomanip>
@@ -142,6 +143,37 @@ void Arena::release() {
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------
+// Ascending sort of (value, location) pairs by value: LSD radix sort on
+// order-preserving 64-bit keys (negative values: all bits flipped; others:
+// sign bit set), one 8-bit digit per pass, passes on a digit that every key
+// shares skipped.  Values are finite.
+struct KeyedRef {
+    uint64_t key;
+    double *ref;
+};
+static void radix_sort(std::vector<KeyedRef> &a, std::vector<KeyedRef> &tmp) {
+    const size_t m = a.size();
+    tmp.resize(m);
+    for (int shift = 0; shift < 64; shift += 8) {
+        size_t cnt[256] = {};
+        for (const KeyedRef &e : a) ++cnt[(e.key >> shift) & 255];
+        if (cnt[(a[0].key >> shift) & 255] == m) continue;
+        size_t off = 0;
+        for (size_t &c : cnt) {
+            const size_t x = c;
+            c = off;
+            off += x;
+        }
+        for (const KeyedRef &e : a) tmp[cnt[(e.key >> shift) & 255]++] = e;
+        a.swap(tmp);
+    }
+}
+static inline uint64_t order_key(double x) {
+    uint64_t u;
+    std::memcpy(&u, &x, sizeof u);
+    return (u >> 63) ? ~u : (u | (1ull << 63));
+}
+
 static Invar add(const Invar &a, const Invar &b) { return {a.first + b.first, a.second + b.second}; }
 static Invar sub(const Invar &a, const Invar &b) { return {a.first - b.first, a.second - b.second}; }
 
@@ -423,20 +455,21 @@ struct Engine {
     // current cluster's first value, and every member takes that first value.
     // One sort of (value, location); the fixed values, still ascending, are
     // kept in it.sorted for the truncation.
+    std::vector<KeyedRef> sort_buf, sort_tmp;
     void fix_splittings(Iter &it) {
-        std::vector<std::pair<double, double *>> e;
+        std::vector<KeyedRef> &e = sort_buf;
+        e.clear();
         for (auto &kv : it.subs)
-            for (double &x : kv.second.value) e.push_back({x, &x});
-        std::sort(e.begin(), e.end(), [](const std::pair<double, double *> &a, const std::pair<double, double *> &b) {
-            return a.first < b.first;
-        });
+            for (double &x : kv.second.value) e.push_back({order_key(x), &x});
+        if (e.empty()) return;
+        radix_sort(e, sort_tmp);
         it.sorted.resize(e.size());
-        size_t i0 = 0;
+        double e0 = *e[0].ref;  // first value of the current cluster
         for (size_t i = 0; i < e.size(); ++i) {
-            if (!(e[i].first - e[i0].first < P.fixeps)) i0 = i;
-            const double rep = e[i0].first;
-            *e[i].second = rep;
-            it.sorted[i] = rep;
+            const double x = *e[i].ref;
+            if (!(x - e0 < P.fixeps)) e0 = x;
+            *e[i].ref = e0;
+            it.sorted[i] = e0;
         }
     }
 
