@@ -1006,20 +1006,33 @@ struct Engine {
             fix_splittings(it);
             truncate_prepare(it, last, false);
             totalenergy += Egs * it.scale;
-            std::map<Invar, std::vector<double>> allv, keptv;
+            // the GPU work of this iteration is queued first (the shifted
+            // values become E_prev, then the recalculations); the host-side
+            // bookkeeping and output follow while it runs
             {
                 std::vector<nrg_strided> items;
                 std::vector<double> all;
                 for (auto &kv : it.subs) {
-                    Sub &s = kv.second;
-                    s.absenergy.resize(s.d);
-                    for (int r = 0; r < s.d; ++r) s.absenergy[r] = totalenergy + s.value[r] * it.scale;
-                    items.push_back({s.W, s.d, 1});  // shifted values become E_prev
-                    all.insert(all.end(), s.value.begin(), s.value.end());
-                    allv[kv.first] = s.value;
-                    keptv[kv.first] = std::vector<double>(s.value.begin(), s.value.begin() + s.kept);
+                    items.push_back({kv.second.W, kv.second.d, 1});
+                    all.insert(all.end(), kv.second.value.begin(), kv.second.value.end());
                 }
                 gpu.scatter(items, all.data());
+            }
+            // ---- recalculation with kept states (strategy=kept)
+            st.old_op_arena = std::move(st.op_arena);
+            st.op_arena.reset(new Arena(&gpu));
+            Arena tmp(&gpu);
+            if (!last) timed("recalc f", [&] { st.f = recalc_f(it, *st.op_arena); });
+            timed("recalc s", [&] {
+                for (auto &kv : st.ops_s) kv.second = recalc_general_all(it, kv.second, false, false, *st.op_arena, tmp);
+            });
+            std::map<Invar, std::vector<double>> allv, keptv;
+            for (auto &kv : it.subs) {
+                Sub &s = kv.second;
+                s.absenergy.resize(s.d);
+                for (int r = 0; r < s.d; ++r) s.absenergy[r] = totalenergy + s.value[r] * it.scale;
+                allv[kv.first] = s.value;
+                keptv[kv.first] = std::vector<double>(s.value.begin(), s.value.begin() + s.kept);
             }
             if (fenergies.is_open()) {
                 fenergies << "\n===== Iteration number: " << N << "\n";
@@ -1040,14 +1053,6 @@ struct Engine {
                 std::cout << "N=" << N << " subspaces=" << it.subs.size() << " Kept: " << nkept << " out of " << nall
                           << " (" << std::chrono::duration<double>(std::chrono::steady_clock::now() - t_iter).count()
                           << " s)" << std::endl;
-            // ---- recalculation with kept states (strategy=kept)
-            st.old_op_arena = std::move(st.op_arena);
-            st.op_arena.reset(new Arena(&gpu));
-            Arena tmp(&gpu);
-            if (!last) timed("recalc f", [&] { st.f = recalc_f(it, *st.op_arena); });
-            timed("recalc s", [&] {
-                for (auto &kv : st.ops_s) kv.second = recalc_general_all(it, kv.second, false, false, *st.op_arena, tmp);
-            });
             custom_row(it.scale / P.betabar, keptv, st.ops_s);
             tmp.release();
             st.old_op_arena.reset();
