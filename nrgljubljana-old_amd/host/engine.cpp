@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <iomanip>
 #include <iostream>
 #include <set>
@@ -292,9 +293,11 @@ struct Engine {
         return s * P.bandrescale;
     }
 
+    // stage timing (a stream sync per stage) only when timed stats are asked for
+    bool stage_timing() const { return !opt.stats.empty() && !opt.stats_light; }
     template <class F>
     void timed(const char *name, F f) {
-        if (opt.stats_light) {  // no stage timing: keep the stream running
+        if (!stage_timing()) {  // keep the stream running
             f();
             return;
         }
@@ -854,6 +857,7 @@ struct Engine {
         }
     }
 
+    std::function<void()> deferred;  // pass 1: host-side output of the previous iteration
     void pass1() {
         State st;
         fresh_state(st);
@@ -983,6 +987,12 @@ struct Engine {
                 } else {
                     gpu.check(nrg_diag_batch(gpu.ctx, tasks.data(), (int)tasks.size()), "nrg_diag_batch");
                 }
+                // the previous iteration's host-side output while the GPU
+                // runs this eigensolve
+                if (deferred) {
+                    deferred();
+                    deferred = nullptr;
+                }
                 std::vector<nrg_strided> items;
                 size_t tot = 0;
                 for (auto &kv : it.subs) {
@@ -1026,36 +1036,46 @@ struct Engine {
             timed("recalc s", [&] {
                 for (auto &kv : st.ops_s) kv.second = recalc_general_all(it, kv.second, false, false, *st.op_arena, tmp);
             });
-            std::map<Invar, std::vector<double>> allv, keptv;
-            for (auto &kv : it.subs) {
-                Sub &s = kv.second;
-                s.absenergy.resize(s.d);
-                for (int r = 0; r < s.d; ++r) s.absenergy[r] = totalenergy + s.value[r] * it.scale;
-                allv[kv.first] = s.value;
-                keptv[kv.first] = std::vector<double>(s.value.begin(), s.value.begin() + s.kept);
-            }
-            if (fenergies.is_open()) {
-                fenergies << "\n===== Iteration number: " << N << "\n";
-                for (auto &kv : allv) {
-                    fenergies << "Subspace: " << kv.first.first << " " << kv.first.second << "\n";
-                    for (double x : kv.second) fenergies << x << ' ';
-                    fenergies << "\n";
+            tmp.release();             // (stream-ordered: later work reuses the memory)
+            st.old_op_arena.reset();   // the previous iteration's operators, read by work already queued
+            // host-side output of this iteration: run once the next
+            // iteration's eigensolve is queued (custom_row then still sees
+            // this iteration's singlet operators), or at once when stages
+            // are timed
+            const double te = totalenergy;
+            auto book = [this, &it, &st, N, te, t_iter] {
+                std::map<Invar, std::vector<double>> allv, keptv;
+                for (auto &kv : it.subs) {
+                    Sub &s = kv.second;
+                    s.absenergy.resize(s.d);
+                    for (int r = 0; r < s.d; ++r) s.absenergy[r] = te + s.value[r] * it.scale;
+                    allv[kv.first] = s.value;
+                    keptv[kv.first] = std::vector<double>(s.value.begin(), s.value.begin() + s.kept);
                 }
-            }
-            write_dump_iter(it);
-            td_row(allv, it.scale);
-            size_t nkept = 0, nall = 0;
-            for (auto &kv : it.subs) {
-                nkept += kv.second.kept;
-                nall += kv.second.d;
-            }
-            if (!opt.quiet)
-                std::cout << "N=" << N << " subspaces=" << it.subs.size() << " Kept: " << nkept << " out of " << nall
-                          << " (" << std::chrono::duration<double>(std::chrono::steady_clock::now() - t_iter).count()
-                          << " s)" << std::endl;
-            custom_row(it.scale / P.betabar, keptv, st.ops_s);
-            tmp.release();
-            st.old_op_arena.reset();
+                if (fenergies.is_open()) {
+                    fenergies << "\n===== Iteration number: " << N << "\n";
+                    for (auto &kv : allv) {
+                        fenergies << "Subspace: " << kv.first.first << " " << kv.first.second << "\n";
+                        for (double x : kv.second) fenergies << x << ' ';
+                        fenergies << "\n";
+                    }
+                }
+                write_dump_iter(it);
+                td_row(allv, it.scale);
+                size_t nkept = 0, nall = 0;
+                for (auto &kv : it.subs) {
+                    nkept += kv.second.kept;
+                    nall += kv.second.d;
+                }
+                if (!opt.quiet)
+                    std::cout << "N=" << N << " subspaces=" << it.subs.size() << " Kept: " << nkept << " out of "
+                              << nall << " ("
+                              << std::chrono::duration<double>(std::chrono::steady_clock::now() - t_iter).count()
+                              << " s)" << std::endl;
+                custom_row(it.scale / P.betabar, keptv, st.ops_s);
+            };
+            if (stage_timing() || last) book();
+            else deferred = book;
             // ---- E_prev for the next iteration: first `kept` values of W
             std::map<Invar, Prev> np;
             for (auto &kv : it.subs) np[kv.first] = Prev{kv.second.kept, kv.second.W};
