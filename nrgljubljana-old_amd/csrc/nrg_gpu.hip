@@ -238,9 +238,10 @@ __global__ __launch_bounds__(256) void k_gemm(const nrg_gemm_out *outs, const nr
         }
         for (int k0 = 0; k0 < tm.k && g < g1; k0 += GK, ++g) {
             if (g < g0) continue;
-            // load A tile: rows row0..+64, k k0..+16 (1024 elements, 4 per thread)
+            // load A tile: rows row0..+GT, k k0..+GK (GT GK / 256 elements per thread)
+            constexpr int LQ = GT * GK / 256;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < LQ; ++q) {
                 const int e = tid + q * 256;
                 int r, kk;
                 if (tm.transA) {  // contiguous along rows of op(A)
@@ -259,7 +260,7 @@ __global__ __launch_bounds__(256) void k_gemm(const nrg_gemm_out *outs, const nr
                 As[kk][r] = v;
             }
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < LQ; ++q) {
                 const int e = tid + q * 256;
                 int cidx, kk;
                 if (tm.transB) {
