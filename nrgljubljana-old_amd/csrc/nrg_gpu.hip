@@ -203,12 +203,68 @@ __global__ __launch_bounds__(256) void k_assemble(const nrg_asm_block *blocks, c
 // ---------------------------------------------------------------------------
 // grouped GEMM (FP64, LDS tiled, 64x64 tile, 4x4 per thread)
 // A workgroup walks the (term, k) sequence of its tile, or of its slice of
-// it, in steps of GK.
+// it, in steps of GK.  Software-pipelined: the global loads of step i+1 are
+// issued right after step i's tile is in LDS and complete while step i is
+// computed; two LDS buffers, one barrier per step.
 // ---------------------------------------------------------------------------
+constexpr int GLQ = GT * GK / 256;  // tile elements per thread and operand
+
+// op(A) rows row0.., k k0.. and op(B) k k0.., columns col0.. of one step
+__device__ __forceinline__ void gemm_load_step(const nrg_gemm_term &tm, int m, int n, int row0, int col0, int k0,
+                                               int tid, double (&ra)[GLQ], double (&rb)[GLQ]) {
+#pragma unroll
+    for (int q = 0; q < GLQ; ++q) {
+        const int e = tid + q * 256;
+        int r, kk;
+        if (tm.transA) {  // contiguous along rows of op(A)
+            kk = e / GT;
+            r = e % GT;
+        } else {
+            r = e / GK;
+            kk = e % GK;
+        }
+        const int gr = row0 + r, gk = k0 + kk;
+        double v = 0.0;
+        if (gr < m && gk < tm.k) {
+            v = tm.transA ? tm.A[(size_t)gk * tm.lda + gr] : tm.A[(size_t)gr * tm.lda + gk];
+            if (tm.s) v *= tm.s[gk];
+        }
+        ra[q] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < GLQ; ++q) {
+        const int e = tid + q * 256;
+        int cidx, kk;
+        if (tm.transB) {
+            cidx = e / GK;
+            kk = e % GK;
+        } else {
+            kk = e / GT;
+            cidx = e % GT;
+        }
+        const int gc = col0 + cidx, gk = k0 + kk;
+        double v = 0.0;
+        if (gc < n && gk < tm.k) v = tm.transB ? tm.B[(size_t)gc * tm.ldb + gk] : tm.B[(size_t)gk * tm.ldb + gc];
+        rb[q] = v * tm.alpha;
+    }
+}
+
+__device__ __forceinline__ void gemm_store_step(const nrg_gemm_term &tm, int tid, const double (&ra)[GLQ],
+                                                const double (&rb)[GLQ], double (*As)[GT + 1], double (*Bs)[GT + 1]) {
+#pragma unroll
+    for (int q = 0; q < GLQ; ++q) {
+        const int e = tid + q * 256;
+        if (tm.transA) As[e / GT][e % GT] = ra[q];
+        else As[e % GK][e / GK] = ra[q];
+        if (tm.transB) Bs[e % GK][e / GK] = rb[q];
+        else Bs[e / GT][e % GT] = rb[q];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_gemm(const nrg_gemm_out *outs, const nrg_gemm_term *terms,
                                               const GemmItem *items, double *part) {
-    __shared__ double As[GK][GT + 1];
-    __shared__ double Bs[GK][GT + 1];
+    __shared__ double As[2][GK][GT + 1];
+    __shared__ double Bs[2][GK][GT + 1];
     const GemmItem it = items[blockIdx.x];
     const nrg_gemm_out o = outs[it.out];
     const int tid = threadIdx.x;
@@ -228,69 +284,59 @@ __global__ __launch_bounds__(256) void k_gemm(const nrg_gemm_out *outs, const nr
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-    int g = 0;
-    for (int t = 0; t < o.term_count && g < g1; ++t) {
-        const nrg_gemm_term tm = terms[o.term_begin + t];
+    // first step of the slice: term t, offset k0
+    int t = 0, k0 = 0, g = 0;
+    nrg_gemm_term tm{};
+    bool have = false;
+    for (; t < o.term_count && g < g1; ++t) {
+        tm = terms[o.term_begin + t];
         const int st = (tm.k + GK - 1) / GK;
         if (g + st <= g0) {  // the whole term lies before this slice
             g += st;
             continue;
         }
-        for (int k0 = 0; k0 < tm.k && g < g1; k0 += GK, ++g) {
-            if (g < g0) continue;
-            // load A tile: rows row0..+GT, k k0..+GK (GT GK / 256 elements per thread)
-            constexpr int LQ = GT * GK / 256;
-#pragma unroll
-            for (int q = 0; q < LQ; ++q) {
-                const int e = tid + q * 256;
-                int r, kk;
-                if (tm.transA) {  // contiguous along rows of op(A)
-                    kk = e / GT;
-                    r = e % GT;
-                } else {
-                    r = e / GK;
-                    kk = e % GK;
+        k0 = (g0 - g) * GK;  // (g <= g0 here)
+        g = g0;
+        have = g0 < g1;  // an empty slice still writes its (zero) partial tile
+        break;
+    }
+    double ra[GLQ], rb[GLQ];
+    if (have) gemm_load_step(tm, o.m, o.n, row0, col0, k0, tid, ra, rb);
+    int buf = 0;
+    while (have) {
+        gemm_store_step(tm, tid, ra, rb, As[buf], Bs[buf]);
+        __syncthreads();
+        // the next step (possibly of a later term), loaded while this one is computed
+        ++g;
+        k0 += GK;
+        bool next = g < g1;
+        if (next && k0 >= tm.k) {
+            next = false;
+            for (++t; t < o.term_count; ++t) {
+                tm = terms[o.term_begin + t];
+                if (tm.k > 0) {
+                    next = true;
+                    k0 = 0;
+                    break;
                 }
-                const int gr = row0 + r, gk = k0 + kk;
-                double v = 0.0;
-                if (gr < o.m && gk < tm.k) {
-                    v = tm.transA ? tm.A[(size_t)gk * tm.lda + gr] : tm.A[(size_t)gr * tm.lda + gk];
-                    if (tm.s) v *= tm.s[gk];
-                }
-                As[kk][r] = v;
             }
-#pragma unroll
-            for (int q = 0; q < LQ; ++q) {
-                const int e = tid + q * 256;
-                int cidx, kk;
-                if (tm.transB) {
-                    cidx = e / GK;
-                    kk = e % GK;
-                } else {
-                    kk = e / GT;
-                    cidx = e % GT;
-                }
-                const int gc = col0 + cidx, gk = k0 + kk;
-                double v = 0.0;
-                if (gc < o.n && gk < tm.k) v = tm.transB ? tm.B[(size_t)gc * tm.ldb + gk] : tm.B[(size_t)gk * tm.ldb + gc];
-                Bs[kk][cidx] = v * tm.alpha;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int kk = 0; kk < GK; ++kk) {
-                double a[4], b[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    a[q] = As[kk][tr + 16 * q];
-                    b[q] = Bs[kk][tc + 16 * q];
-                }
-#pragma unroll
-                for (int x = 0; x < 4; ++x)
-#pragma unroll
-                    for (int y = 0; y < 4; ++y) acc[x][y] = fma(a[x], b[y], acc[x][y]);
-            }
-            __syncthreads();
         }
+        if (next) gemm_load_step(tm, o.m, o.n, row0, col0, k0, tid, ra, rb);
+#pragma unroll
+        for (int kk = 0; kk < GK; ++kk) {
+            double a[4], b[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a[q] = As[buf][kk][tr + 16 * q];
+                b[q] = Bs[buf][kk][tc + 16 * q];
+            }
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+        }
+        buf ^= 1;
+        have = next;
     }
     if (it.ns > 1) {
         double *pt = part + (size_t)(it.part + it.s) * GT * GT;
