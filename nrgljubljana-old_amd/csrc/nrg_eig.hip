@@ -384,12 +384,14 @@ __device__ __forceinline__ void tri_team(const EigMat &mt, int rank, int T, int 
 // ---------------------------------------------------------------------------
 constexpr int REG_ONE_N = 128;  // one workgroup: 16 waves x 8 rows, 2 column registers
 constexpr int REG_MAX_N = 576;  // largest team variant: 9 column registers
-constexpr int REG_NT = 512;     // team workgroups: 8 waves
+constexpr int REG_NT = 512;     // team workgroups: 8 waves, two per SIMD (one per SIMD exposes the
+                                // latency of each step's reduction chain: 12% slower)
+constexpr int REG_RPW[3] = {8, 4, 3};  // rows per wavefront of the d <= 256 / 384 / 576 variants
 
 // team size of the register variant for dimension n (REG_ONE_N < n <=
 // REG_MAX_N): workgroups of 8 waves x (8, 4, 3) rows
 __host__ __device__ inline int reg_team(int n) {
-    const int rows = (REG_NT / 64) * (n <= 256 ? 8 : n <= 384 ? 4 : 3);
+    const int rows = (REG_NT / 64) * (n <= 256 ? REG_RPW[0] : n <= 384 ? REG_RPW[1] : REG_RPW[2]);
     return (n + rows - 1) / rows;
 }
 
@@ -680,9 +682,9 @@ __global__ __launch_bounds__(REG_NT) void k_eig_tri_reg(const EigMat *mats, cons
         const TriJob jb = jobs[lo];
         const EigMat mt = mats[jb.mat];
         const int rank = t0 + t - tbase[lo];
-        if (mt.n <= 256) tri_reg<4, 8, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf, vsh);
-        else if (mt.n <= 384) tri_reg<6, 4, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf, vsh);
-        else tri_reg<9, 3, REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf, vsh);
+        if (mt.n <= 256) tri_reg<4, REG_RPW[0], REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf, vsh);
+        else if (mt.n <= 384) tri_reg<6, REG_RPW[1], REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf, vsh);
+        else tri_reg<9, REG_RPW[2], REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf, vsh);
         __syncthreads();
     }
 }
@@ -2050,7 +2052,8 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_coop, TRI_NT, TRI_COOP_LDS));
         c->tri_capacity = std::max(1, per_cu * ncu);
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_reg, REG_NT, 0));
-        c->reg_capacity = std::max(1, per_cu * ncu);
+        // at most one team workgroup per CU: two would share its SIMDs
+        c->reg_capacity = std::max(1, std::min(per_cu, 1) * ncu);
         HIPCHK(c, hipFuncSetAttribute((const void *)k_dc_small, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)sizeof(DcSmall)));
     }
