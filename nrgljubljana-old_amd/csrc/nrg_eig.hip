@@ -26,6 +26,8 @@
 //                of the result; larger n: compact WY blocks and GEMMs.
 // Output convention as diagonalise(): eigenvalues ascending in W, row r of A
 // = eigenvector r.
+#include <type_traits>
+
 #include "nrg_internal.h"
 
 constexpr int TRI_NT = 1024;
@@ -386,10 +388,14 @@ constexpr int REG_ONE_N = 128;  // one workgroup: 16 waves x 8 rows, 2 column re
 constexpr int REG_MAX_N = 576;  // largest team variant: 9 column registers
 constexpr int REG_NT = 512;     // team workgroups: 8 waves, two per SIMD (one per SIMD exposes the
                                 // latency of each step's reduction chain: 12% slower)
-constexpr int REG_RPW[3] = {8, 4, 3};  // rows per wavefront of the d <= 256 / 384 / 576 variants
+// rows per wavefront of the d <= 256 / 384 / 576 variants: few rows keep the
+// per-step row reductions short (more workgroups per team, up to 24 of the
+// 32 CUs of an XCD at d = 384 and 576); one row at d <= 256 leaves too few
+// CUs to the one-workgroup matrices on the side stream
+constexpr int REG_RPW[3] = {2, 2, 3};
 
 // team size of the register variant for dimension n (REG_ONE_N < n <=
-// REG_MAX_N): workgroups of 8 waves x (8, 4, 3) rows
+// REG_MAX_N): workgroups of 8 waves x REG_RPW rows
 __host__ __device__ inline int reg_team(int n) {
     const int rows = (REG_NT / 64) * (n <= 256 ? REG_RPW[0] : n <= 384 ? REG_RPW[1] : REG_RPW[2]);
     return (n + rows - 1) / rows;
@@ -401,6 +407,19 @@ template <int R>
 __device__ __forceinline__ void wave_sum_n(double (&v)[R]) {
 #pragma unroll
     for (int s = 0; s < R; ++s) v[s] = wave_sum(v[s]);
+}
+
+// Calls f(integral_constant<P>, j) with P the largest multiple of PS not
+// above the run-time p (0 <= p < NC): at most ceil(NC / PS) copies of the
+// step, few enough to keep the register allocation of one.
+template <int P, int PS, int NC, class F>
+__device__ __forceinline__ bool tri_reg_dispatch(int p, F &f, int j) {
+    if constexpr (P + PS >= NC) {
+        return f(std::integral_constant<int, P>{}, j);
+    } else {
+        if (p < P + PS) return f(std::integral_constant<int, P>{}, j);
+        return tri_reg_dispatch<P + PS, PS, NC>(p, f, j);
+    }
 }
 
 template <int NC, int RPW, int NW, bool TEAM>
@@ -452,11 +471,15 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
     for (int s = 0; s < RPW; ++s) vrow[s] = 0.0;
     double tj = 0.0;
     int phase = 0;
-    for (int j = -1; j <= n - 3; ++j) {
+    // step j; registers below (j + 1) / 64 hold only finished columns
+    // (<= j), whose operands are all zero: the work of those below P is
+    // skipped
+    auto step = [&](auto Pc, const int j) -> bool {
+        constexpr int P = decltype(Pc)::value;
         double w[NC], xr[NC], w1 = 0.0, wrow[RPW], x0, x1;
         if (j < 0) {
 #pragma unroll
-            for (int i = 0; i < NC; ++i) {
+            for (int i = P; i < NC; ++i) {
                 const int c = lane + 64 * i;
                 const double x = A[min(c, n - 1)];
                 xr[i] = c < n ? x : 0.0;
@@ -496,7 +519,7 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
             }
             double pp = 0.0;
 #pragma unroll
-            for (int i = 0; i < NC; ++i) {
+            for (int i = P; i < NC; ++i) {
                 const int c = lane + 64 * i;
                 if constexpr (TEAM) {
                     w[i] = pg[c];
@@ -525,12 +548,12 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
             }
             const double K = 0.5 * tj * wave_sum(pp);
 #pragma unroll
-            for (int i = 0; i < NC; ++i) w[i] -= K * vv[i];
+            for (int i = P; i < NC; ++i) w[i] -= K * vv[i];
             w1 = p1 - K * v1;
 #pragma unroll
             for (int s = 0; s < RPW; ++s) wrow[s] = prow[s] - K * vrow[s];
 #pragma unroll
-            for (int i = 0; i < NC; ++i) xr[i] -= v1 * w[i] + w1 * vv[i];
+            for (int i = P; i < NC; ++i) xr[i] -= v1 * w[i] + w1 * vv[i];
             // row j+1 at columns j+1 and j+2 from the LDS copies (the same
             // inputs in every wavefront; these two entries are used only for
             // the diagonal and the next reflector's pivot)
@@ -545,13 +568,13 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
 #pragma unroll
         for (int s = 0; s < RPW; ++s) {
 #pragma unroll
-            for (int i = 0; i < NC; ++i) a[s][i] -= vrow[s] * w[i] + wrow[s] * vv[i];
+            for (int i = P; i < NC; ++i) a[s][i] -= vrow[s] * w[i] + wrow[s] * vv[i];
         }
         // row j+1 is final: diagonal x0, and the reflector that zeroes it
         // beyond column j+2 (branch-free: a uniform choice of results)
         double ps = 0.0;
 #pragma unroll
-        for (int i = 0; i < NC; ++i) {
+        for (int i = P; i < NC; ++i) {
             const int c = lane + 64 * i;
             if (c > j + 2 && (TEAM || c < n)) ps += xr[i] * xr[i];  // (team rows are zero past n)
         }
@@ -566,7 +589,9 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
         const bool live = tn != 0.0;
         double nv[NC];
 #pragma unroll
-        for (int i = 0; i < NC; ++i) {
+        for (int i = 0; i < P; ++i) nv[i] = 0.0;
+#pragma unroll
+        for (int i = P; i < NC; ++i) {
             const int c = lane + 64 * i;
             nv[i] = !live ? 0.0 : (c == j + 2) ? v0 : ((c > j + 2 && (TEAM || c < n)) ? xr[i] : 0.0);
         }
@@ -577,7 +602,7 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
         for (int s = 0; s < RPW; ++s) {
             acc[s] = 0.0;
 #pragma unroll
-            for (int i = 0; i < NC; ++i) acc[s] += a[s][i] * nv[i];
+            for (int i = P; i < NC; ++i) acc[s] += a[s][i] * nv[i];
         }
         wave_sum_n<RPW>(acc);
         double *pn = ((j + 3) & 1) ? pb1 : pb0, *rn = ((j + 3) & 1) ? rb1 : rb0;
@@ -591,7 +616,7 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
                 }
                 if (r == j + 2) {
 #pragma unroll
-                    for (int i = 0; i < NC; ++i) {
+                    for (int i = P; i < NC; ++i) {
                         const int c = lane + 64 * i;
                         if (c >= j + 2 && c < n) {
                             if (TEAM) st_dev(rn + c, a[s][i]);
@@ -606,7 +631,7 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
         if (lead) {
             if (j >= 0) {
 #pragma unroll
-                for (int i = 0; i < NC; ++i) {
+                for (int i = P; i < NC; ++i) {
                     const int c = lane + 64 * i;
                     if (c > j && c < n) A[(size_t)j * n + c] = vv[i];
                 }
@@ -620,7 +645,7 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
         // v_{j+1} becomes current; wavefront 0 leaves it in LDS for the
         // entries at the own rows (read after the step's barrier)
 #pragma unroll
-        for (int i = 0; i < NC; ++i) vv[i] = nv[i];
+        for (int i = P; i < NC; ++i) vv[i] = nv[i];
         v1 = live ? v0 : 0.0;
         if (wave == 0) {
 #pragma unroll
@@ -629,11 +654,15 @@ __device__ __forceinline__ void tri_reg(const EigMat &mt, int rank, int T, int *
         tj = tn;
         if (TEAM) {
             ++phase;
-            if (!team_barrier(bar, phase * T, status, s_flag)) return;
+            if (!team_barrier(bar, phase * T, status, s_flag)) return false;
         } else {
             __syncthreads();
         }
-    }
+            return true;
+    };
+    for (int j = -1; j <= n - 3; ++j)
+        if (!tri_reg_dispatch<0, TEAM ? (NC + 2) / 3 : NC, NC>((j + 1) >> 6, step, j)) return;
+
     if (rank == 0 && tid == 0) {
         const double *rl = (n & 1) ? rb1 : rb0;  // row n-1, published by the last step
         gdg[n - 1] = TEAM ? ld_dev(rl + n - 1) : rl[n - 1];
