@@ -173,27 +173,49 @@ static nrg_status xfer_prepare(nrg_ctx *c, const nrg_strided *items, int nitems,
 // ---------------------------------------------------------------------------
 // assembly
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_assemble(const nrg_asm_block *blocks, const nrg_asm_term *terms) {
-    const nrg_asm_block b = blocks[blockIdx.x];
-    const int d = b.d;
+// One workgroup per band of ASM_BAND rows of a block: the band is zeroed,
+// then every term adds what falls into its rows, in term order (for a block
+// term its direct part, then its transposed part), so each element sums the
+// same contributions in the same order as a whole-block pass would.
+constexpr int ASM_BAND = 16;
+struct AsmItem {
+    int block, r0, r1, pad;  // rows [r0, r1) of block `block`
+};
+
+__global__ __launch_bounds__(256) void k_assemble(const nrg_asm_block *blocks, const nrg_asm_term *terms,
+                                                  const AsmItem *items) {
+    const AsmItem it = items[blockIdx.x];
+    const nrg_asm_block b = blocks[it.block];
+    const int d = b.d, R0 = it.r0, R1 = it.r1;
     double *H = b.H;
-    const size_t dd = (size_t)d * d;
-    for (size_t i = threadIdx.x; i < dd; i += blockDim.x) H[i] = 0.0;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < (R1 - R0) * d; i += blockDim.x) H[(size_t)R0 * d + i] = 0.0;
     __syncthreads();
     for (int t = 0; t < b.term_count; ++t) {
         const nrg_asm_term tm = terms[b.term_begin + t];
-        if (tm.kind == NRG_TERM_DIAGVEC) {
-            for (int r = threadIdx.x; r < tm.rows; r += blockDim.x)
-                H[(size_t)(tm.r0 + r) * d + tm.r0 + r] += tm.scale * tm.src[r];
-        } else if (tm.kind == NRG_TERM_SHIFT) {
-            for (int r = threadIdx.x; r < tm.rows; r += blockDim.x) H[(size_t)(tm.r0 + r) * d + tm.r0 + r] += tm.scale;
+        if (tm.kind == NRG_TERM_DIAGVEC || tm.kind == NRG_TERM_SHIFT) {
+            const int lo = max(R0, tm.r0), hi = min(R1, tm.r0 + tm.rows);
+            for (int r = lo + tid; r < hi; r += blockDim.x)
+                H[(size_t)r * d + r] += tm.kind == NRG_TERM_DIAGVEC ? tm.scale * tm.src[r - tm.r0] : tm.scale;
         } else {
-            const int n = tm.rows * tm.cols;
-            for (int e = threadIdx.x; e < n; e += blockDim.x) {
-                const int r = e / tm.cols, cc = e - r * tm.cols;
-                const double v = tm.scale * tm.src[(size_t)r * tm.ld + cc];
-                H[(size_t)(tm.r0 + r) * d + tm.c0 + cc] += v;
-                H[(size_t)(tm.c0 + cc) * d + tm.r0 + r] += v;
+            // direct part: rows r0 + r of the band, columns c0 + cc
+            const int lo = max(R0, tm.r0), hi = min(R1, tm.r0 + tm.rows);
+            if (lo < hi) {
+                const int n = (hi - lo) * tm.cols;
+                for (int e = tid; e < n; e += blockDim.x) {
+                    const int r = lo - tm.r0 + e / tm.cols, cc = e % tm.cols;
+                    H[(size_t)(tm.r0 + r) * d + tm.c0 + cc] += tm.scale * tm.src[(size_t)r * tm.ld + cc];
+                }
+            }
+            __syncthreads();
+            // transposed part: rows c0 + cc of the band, columns r0 + r
+            const int clo = max(R0, tm.c0), chi = min(R1, tm.c0 + tm.cols);
+            if (clo < chi) {
+                const int n = (chi - clo) * tm.rows;
+                for (int e = tid; e < n; e += blockDim.x) {
+                    const int cc = clo - tm.c0 + e / tm.rows, r = e % tm.rows;
+                    H[(size_t)(tm.c0 + cc) * d + tm.r0 + r] += tm.scale * tm.src[(size_t)r * tm.ld + cc];
+                }
             }
         }
         __syncthreads();
@@ -717,18 +739,24 @@ nrg_status nrg_assemble_batch(nrg_ctx *c, const nrg_asm_block *blocks, int nbloc
                 return fail(c, NRG_ERR_ARG, "assemble: term out of range");
         }
     }
-    const size_t bb = sizeof(nrg_asm_block) * nblocks, tb = sizeof(nrg_asm_term) * nterms;
-    std::vector<char> buf(bb + tb);
+    std::vector<AsmItem> items;
+    for (int b = 0; b < nblocks; ++b)
+        for (int r0 = 0; r0 < blocks[b].d; r0 += ASM_BAND) items.push_back({b, r0, std::min(r0 + ASM_BAND, blocks[b].d), 0});
+    const size_t bb = sizeof(nrg_asm_block) * nblocks, tb = sizeof(nrg_asm_term) * nterms,
+                 ib = sizeof(AsmItem) * items.size();
+    const size_t to = (bb + 15) & ~(size_t)15, io = (to + tb + 15) & ~(size_t)15;
+    std::vector<char> buf(io + ib);
     memcpy(buf.data(), blocks, bb);
-    if (tb) memcpy(buf.data() + bb, terms, tb);
+    if (tb) memcpy(buf.data() + to, terms, tb);
+    memcpy(buf.data() + io, items.data(), ib);
     void *d;
     nrg_status s = stage(c, buf.data(), buf.size(), &d);
     if (s) return s;
     double abytes = 0;
     for (int b = 0; b < nblocks; ++b) abytes += 8.0 * blocks[b].d * blocks[b].d;
     LaunchTimer lt(K_ASSEMBLE, c->stream, 0.0, abytes);
-    hipLaunchKernelGGL(k_assemble, dim3(nblocks), dim3(256), 0, c->stream, (const nrg_asm_block *)d,
-                       (const nrg_asm_term *)((char *)d + bb));
+    hipLaunchKernelGGL(k_assemble, dim3(items.size()), dim3(256), 0, c->stream, (const nrg_asm_block *)d,
+                       (const nrg_asm_term *)((char *)d + to), (const AsmItem *)((char *)d + io));
     HIPCHK(c, hipGetLastError());
     return NRG_OK;
 }
