@@ -1706,6 +1706,18 @@ __global__ __launch_bounds__(64 * RANK_NW) void k_dc_rank(const EigMat *mats, co
 //    vector in registers (NS x 64 elements), reflectors read from A.
 //    Output: row rank(c) of S1.
 // ---------------------------------------------------------------------------
+// Calls f(integral_constant<P>) with P the largest multiple of PS not above
+// the run-time p (0 <= p < NS).
+template <int P, int PS, int NS, class F>
+__device__ __forceinline__ void bt_dispatch(int p, F &f) {
+    if constexpr (P + PS >= NS) {
+        f(std::integral_constant<int, P>{});
+    } else {
+        if (p < P + PS) f(std::integral_constant<int, P>{});
+        else bt_dispatch<P + PS, PS, NS>(p, f);
+    }
+}
+
 // BT_NT threads: eight eigenvectors per workgroup, one wavefront each; the
 // reflectors are staged BT_RB at a time in LDS (read once per workgroup
 // instead of once per eigenvector) and applied from there.
@@ -1727,29 +1739,56 @@ __global__ __launch_bounds__(BT_NT) void k_eig_bt(const EigMat *mats, const BtIt
         const int r = 64 * k + lane;
         x[k] = (act && r < n) ? ws[(size_t)c * n + r] : 0.0;
     }
+    // batches of reflectors j0, j0-1, .., j0-nb+1; the next batch is loaded
+    // into registers while the current one is applied
+    constexpr int LQ = BT_RB * 64 * NS / BT_NT;
+    static_assert(LQ * BT_NT == BT_RB * 64 * NS, "whole batches per thread");
+    double nxt[LQ];
+    auto load_batch = [&](int j0) {
+        const int nb = min(BT_RB, j0 + 1);
+#pragma unroll
+        for (int q = 0; q < LQ; ++q) {
+            const int e = threadIdx.x + q * BT_NT;
+            const int b = e / (64 * NS), r = e - b * 64 * NS, j = j0 - b;
+            const bool ok = b < nb && r > j && r < n;
+            const double v = A[ok ? (size_t)j * n + r : 0];
+            nxt[q] = ok ? v : 0.0;
+        }
+    };
+    if (n >= 3) load_batch(n - 3);
     for (int j0 = n - 3; j0 >= 0; j0 -= BT_RB) {
         const int nb = min(BT_RB, j0 + 1);  // reflectors j0, j0-1, .., j0-nb+1
         __syncthreads();                    // the previous batch has been applied
-        for (int e = threadIdx.x; e < nb * 64 * NS; e += BT_NT) {
-            const int b = e / (64 * NS), r = e - b * 64 * NS, j = j0 - b;
-            refl[b][r] = (r > j && r < n) ? A[(size_t)j * n + r] : 0.0;
+#pragma unroll
+        for (int q = 0; q < LQ; ++q) {
+            const int e = threadIdx.x + q * BT_NT;
+            refl[e / (64 * NS)][e % (64 * NS)] = nxt[q];
         }
         __syncthreads();
+        if (j0 - BT_RB >= 0) load_batch(j0 - BT_RB);
         if (!act) continue;
-        for (int b = 0; b < nb; ++b) {
-            const double t = tau[j0 - b];
-            if (t == 0.0) continue;
-            double vr[NS];
-            double dot = 0.0;
+        // reflectors j >= j0 - nb + 1 are zero in rows <= j: registers below
+        // P are untouched (their products are exact zeros), so their work
+        // is skipped
+        auto apply = [&](auto Pc) {
+            constexpr int P = decltype(Pc)::value;
+            for (int b = 0; b < nb; ++b) {
+                const double t = tau[j0 - b];
+                if (t == 0.0) continue;
+                double vr[NS];
+                double dot = 0.0;
 #pragma unroll
-            for (int k = 0; k < NS; ++k) {
-                vr[k] = refl[b][64 * k + lane];
-                dot += vr[k] * x[k];
+                for (int k = P; k < NS; ++k) {
+                    vr[k] = refl[b][64 * k + lane];
+                    dot += vr[k] * x[k];
+                }
+                dot = wave_sum(dot) * t;
+#pragma unroll
+                for (int k = P; k < NS; ++k) x[k] -= dot * vr[k];
             }
-            dot = wave_sum(dot) * t;
-#pragma unroll
-            for (int k = 0; k < NS; ++k) x[k] -= dot * vr[k];
-        }
+            return true;
+        };
+        bt_dispatch<0, (NS + 3) / 4, NS>((j0 - nb + 2) >> 6, apply);
     }
     if (!act) return;
     const int rk = islot(ws, n, V_RANK)[c];
