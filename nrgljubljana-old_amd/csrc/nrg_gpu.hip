@@ -223,13 +223,15 @@ __global__ __launch_bounds__(256) void k_assemble(const nrg_asm_block *blocks, c
 }
 
 // ---------------------------------------------------------------------------
-// grouped GEMM (FP64, LDS tiled, 64x64 tile, 4x4 per thread)
+// grouped GEMM (FP64, LDS tiled, GT x GT tile, GMT x GMT per thread; 32 x 32:
+// the stage launches have few tiles, so small tiles buy parallelism)
 // A workgroup walks the (term, k) sequence of its tile, or of its slice of
 // it, in steps of GK.  Software-pipelined: the global loads of step i+1 are
 // issued right after step i's tile is in LDS and complete while step i is
 // computed; two LDS buffers, one barrier per step.
 // ---------------------------------------------------------------------------
 constexpr int GLQ = GT * GK / 256;  // tile elements per thread and operand
+constexpr int GMT = GT / 16;        // outputs per thread along each tile dimension
 
 // op(A) rows row0.., k k0.. and op(B) k k0.., columns col0.. of one step
 __device__ __forceinline__ void gemm_load_step(const nrg_gemm_term &tm, int m, int n, int row0, int col0, int k0,
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(256) void k_gemm(const nrg_gemm_out *outs, const nr
     const GemmItem it = items[blockIdx.x];
     const nrg_gemm_out o = outs[it.out];
     const int tid = threadIdx.x;
-    const int tr = tid / 16, tc = tid % 16;  // 16x16 threads, 4x4 each
+    const int tr = tid / 16, tc = tid % 16;  // 16x16 threads, GMT x GMT each
     const int row0 = it.tm * GT, col0 = it.tn * GT;
     if (row0 >= o.m || col0 >= o.n) return;  // m may have been lowered on the device
     // k-steps [g0, g1) of the output's (term, k) sequence belong to this item
@@ -301,11 +303,11 @@ __global__ __launch_bounds__(256) void k_gemm(const nrg_gemm_out *outs, const nr
         g0 = (int)((long long)S * it.s / it.ns);
         g1 = (int)((long long)S * (it.s + 1) / it.ns);
     }
-    double acc[4][4];
+    double acc[GMT][GMT];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < GMT; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+        for (int b = 0; b < GMT; ++b) acc[a][b] = 0.0;
     // first step of the slice: term t, offset k0
     int t = 0, k0 = 0, g = 0;
     nrg_gemm_term tm{};
@@ -346,16 +348,16 @@ __global__ __launch_bounds__(256) void k_gemm(const nrg_gemm_out *outs, const nr
         if (next) gemm_load_step(tm, o.m, o.n, row0, col0, k0, tid, ra, rb);
 #pragma unroll
         for (int kk = 0; kk < GK; ++kk) {
-            double a[4], b[4];
+            double a[GMT], b[GMT];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < GMT; ++q) {
                 a[q] = As[buf][kk][tr + 16 * q];
                 b[q] = Bs[buf][kk][tc + 16 * q];
             }
 #pragma unroll
-            for (int x = 0; x < 4; ++x)
+            for (int x = 0; x < GMT; ++x)
 #pragma unroll
-                for (int y = 0; y < 4; ++y) acc[x][y] = fma(a[x], b[y], acc[x][y]);
+                for (int y = 0; y < GMT; ++y) acc[x][y] = fma(a[x], b[y], acc[x][y]);
         }
         buf ^= 1;
         have = next;
@@ -363,17 +365,17 @@ __global__ __launch_bounds__(256) void k_gemm(const nrg_gemm_out *outs, const nr
     if (it.ns > 1) {
         double *pt = part + (size_t)(it.part + it.s) * GT * GT;
 #pragma unroll
-        for (int x = 0; x < 4; ++x)
+        for (int x = 0; x < GMT; ++x)
 #pragma unroll
-            for (int y = 0; y < 4; ++y) pt[(tr + 16 * x) * GT + tc + 16 * y] = acc[x][y];
+            for (int y = 0; y < GMT; ++y) pt[(tr + 16 * x) * GT + tc + 16 * y] = acc[x][y];
         return;
     }
 #pragma unroll
-    for (int x = 0; x < 4; ++x) {
+    for (int x = 0; x < GMT; ++x) {
         const int gr = row0 + tr + 16 * x;
         if (gr >= o.m) continue;
 #pragma unroll
-        for (int y = 0; y < 4; ++y) {
+        for (int y = 0; y < GMT; ++y) {
             const int gc = col0 + tc + 16 * y;
             if (gc >= o.n) continue;
             double *cp = o.C + (size_t)gr * o.ldc + gc;

@@ -206,12 +206,12 @@ __device__ inline double block_max(double v, double *red) {
 }
 
 // ---------------------------------------------------------------------------
-// grouped GEMM work items: one 64x64 output tile, or slice s of ns of its
+// grouped GEMM work items: one GT x GT output tile, or slice s of ns of its
 // (term, k) sequence.  Slices write partial tiles (slot part + s of the
 // context's partial buffer); one reduce item per split tile then adds them
 // in slice order into C, so results do not depend on scheduling.
 // ---------------------------------------------------------------------------
-constexpr int GT = 64, GK = 16;
+constexpr int GT = 32, GK = 16;
 struct GemmItem {
     int out, tm, tn;
     int s, ns, part;  // ns <= 1: the tile is written directly
