@@ -19,6 +19,7 @@
 #include <iostream>
 #include <set>
 #include <sstream>
+#include <thread>
 
 #include <mutex>
 
@@ -1580,8 +1581,12 @@ struct Engine {
             const double bmin = P.broaden_min <= 0.0 ? P.broaden_min_ratio * SCALE(Nmax) : P.broaden_min;
             std::vector<double> mesh;
             for (double E = P.broaden_max; E > bmin; E /= P.broaden_ratio) mesh.push_back(E);
-            std::vector<std::pair<double, double>> dpos, dneg;
-            for (double E : mesh) {
+            // every mesh point sums over all bins (a log and two exps per
+            // term): the points are independent and spread over host threads,
+            // each summed in the same order as a serial loop would
+            std::vector<std::pair<double, double>> dpos(mesh.size()), dneg(mesh.size());
+            auto point = [&](size_t k) {
+                const double E = mesh[k];
                 double vp = 0, vn = 0;
                 for (size_t i = 0; i < ep.size(); ++i) {
                     vp += wp[i] * BR_NEW(E, ep[i]);
@@ -1591,9 +1596,18 @@ struct Engine {
                     vn += wq[i] * BR_NEW(-E, -en[i]);
                     vp += wq[i] * BR_NEW(E, -en[i]);
                 }
-                dpos.push_back({E, vp});
-                dneg.push_back({-E, vn});
-            }
+                dpos[k] = {E, vp};
+                dneg[k] = {-E, vn};
+            };
+            const size_t nth = std::min<size_t>(mesh.size() / 16 + 1,
+                                                std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+            std::vector<std::thread> pool;
+            for (size_t t = 1; t < nth; ++t)
+                pool.emplace_back([&, t] {
+                    for (size_t k = t; k < mesh.size(); k += nth) point(k);
+                });
+            for (size_t k = 0; k < mesh.size(); k += nth) point(k);
+            for (auto &th : pool) th.join();
             std::sort(dpos.begin(), dpos.end());
             std::sort(dneg.begin(), dneg.end());
             std::ofstream f(fn + ".dat");
