@@ -144,6 +144,16 @@ typedef struct {
 
 nrg_status nrg_diag_batch(nrg_ctx *ctx, const nrg_eig_task *tasks, int ntasks);
 
+/* No counterpart in the reference (diagonalise() returns values and vectors
+ * together, src/diag.h:297-353): after nrg_diag_batch, waits until that
+ * batch's eigenvalues are final — before its eigenvectors, whose
+ * back-transformation is still running — and copies them to host, the W of
+ * each task concatenated in the order the tasks were passed (count = sum of
+ * the n).  The caller can truncate (nrg_truncate_prepare, src/nrg.cc:1919-1998)
+ * while the GPU finishes the eigenvectors.  Reports the batch's device-side
+ * status like a wait; NRG_ERR_ARG without a queued batch or for another count. */
+nrg_status nrg_diag_values(nrg_ctx *ctx, double *host, size_t count);
+
 /* ---- grouped GEMM ------------------------------------------------------- */
 /* Replaces the dgemm calls of recalc_f (src/recalc.cc:90-91), recalc_general
  * (src/recalc.cc:268-272) and cdmI (src/dmnrg.h:382-387) for a whole

@@ -138,6 +138,13 @@ __global__ void k_status_or(const int *src, int *dst) {
     if (threadIdx.x == 0 && *src) *dst = *src;
 }
 
+// eigenvalues of matrix k (final once ranked) to dst + voff[k] (pinned host
+// memory, in the caller's task order)
+__global__ __launch_bounds__(256) void k_values_out(const EigMat *mats, const long long *voff, double *dst) {
+    const EigMat mt = mats[blockIdx.x];
+    for (int r = threadIdx.x; r < mt.n; r += blockDim.x) dst[voff[blockIdx.x] + r] = mt.W[r];
+}
+
 // ---------------------------------------------------------------------------
 // 1b. team tridiagonalisation: T workgroups share one matrix; row r of A
 //     belongs to rank r mod T for the whole reduction (cyclic, so the active
@@ -1916,8 +1923,25 @@ struct HostArena {
 extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, int ntasks) {
     if (ntasks <= 0) return NRG_OK;
     // largest first: the LDS-sized tail of the tridiagonalisation is contiguous
-    std::vector<nrg_eig_task> tasks(tasks_in, tasks_in + ntasks);
-    std::stable_sort(tasks.begin(), tasks.end(), [](const nrg_eig_task &a, const nrg_eig_task &b) { return a.n > b.n; });
+    std::vector<int> by_size(ntasks);
+    for (int i = 0; i < ntasks; ++i) by_size[i] = i;
+    std::stable_sort(by_size.begin(), by_size.end(), [&](int a, int b) { return tasks_in[a].n > tasks_in[b].n; });
+    std::vector<nrg_eig_task> tasks(ntasks);
+    for (int i = 0; i < ntasks; ++i) tasks[i] = tasks_in[by_size[i]];
+    // eigenvalues for nrg_diag_values, concatenated in the caller's task order
+    std::vector<long long> voff_in(ntasks + 1, 0);
+    for (int i = 0; i < ntasks; ++i) voff_in[i + 1] = voff_in[i] + std::max(0, (int)tasks_in[i].n);
+    std::vector<long long> voff(ntasks);
+    for (int i = 0; i < ntasks; ++i) voff[i] = voff_in[by_size[i]];
+    const size_t nvals = (size_t)voff_in[ntasks];
+    if (nvals > c->hval_doubles) {  // (no queued copy may still write the old buffer)
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->hval) HIPCHK(c, hipHostFree(c->hval));
+        c->hval = nullptr;
+        const size_t nd = std::max<size_t>(nvals * 2, 1 << 14);
+        HIPCHK(c, hipHostMalloc((void **)&c->hval, nd * sizeof(double), hipHostMallocDefault));
+        c->hval_doubles = nd;
+    }
     size_t tot = 0;
     std::vector<size_t> off(ntasks);
     for (int i = 0; i < ntasks; ++i) {
@@ -2220,7 +2244,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     HostArena ar;
     const size_t o_jobs = ar.put(jobs), o_tb = ar.put(tbase), o_qj = ar.put(qjob), o_ones = ar.put(ones);
     const size_t o_rjobs = ar.put(rjobs), o_rtb = ar.put(rtbase), o_rqj = ar.put(rqjob);
-    const size_t o_small = ar.put(small);
+    const size_t o_small = ar.put(small), o_voff = ar.put(voff);
     const size_t o_mats = ar.put(mats), o_merges = ar.put(merges), o_sch = ar.put(schunks), o_uch = ar.put(uchunks),
                  o_xch = ar.put(gathers), o_cpy = ar.put(copies), o_gouts = ar.put(gouts), o_gterms = ar.put(gterms), o_gitems = ar.put(gitems), o_gred = ar.put(gred), o_rk = ar.put(rk),
                  o_cp = ar.put(cp);
@@ -2294,7 +2318,22 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         if (!ones.empty()) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     }
     HIPCHK(c, hipGetLastError());
-    if (nbig == 0) return NRG_OK;
+    // once the eigenvalues are final: the status word (set only by the
+    // tridiagonalisation teams) and the eigenvalues go to pinned memory, and
+    // ev_values marks them for nrg_diag_values, before the eigenvectors
+    auto values_out = [&]() -> nrg_status {
+        if (!c->status_pending) *c->hstat = 0;  // nothing queued writes it now
+        hipLaunchKernelGGL(k_status_or, dim3(1), dim3(64), 0, c->stream, c->dstatus, c->hstat);
+        c->status_pending = true;
+        hipLaunchKernelGGL(k_values_out, dim3(ntasks), dim3(256), 0, c->stream, d_mats,
+                           (const long long *)(db + o_voff), c->hval);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->ev_values, c->stream));
+        c->values_pending = true;
+        c->values_count = nvals;
+        return NRG_OK;
+    };
+    if (nbig == 0) return values_out();
     // 2. divide and conquer
     {
         LaunchTimer lt(K_DC_LEAF, c->stream, 0.0, 0.0);
@@ -2353,6 +2392,8 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         hipLaunchKernelGGL(k_dc_rank, dim3(rk.size()), dim3(64 * RANK_NW), 0, c->stream, d_mats, (const BtItem *)(db + o_rk));
     }
     HIPCHK(c, hipGetLastError());
+    s = values_out();
+    if (s) return s;
     {
         double fl = 0, by = 0;
         for (int i = 0; i < nbig; ++i) {
@@ -2430,11 +2471,5 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         }
         HIPCHK(c, hipGetLastError());
     }
-    // the status word reaches the host through pinned memory (no copy) and
-    // is read by the next wait on the stream
-    if (!c->status_pending) *c->hstat = 0;  // nothing queued writes it now
-    hipLaunchKernelGGL(k_status_or, dim3(1), dim3(64), 0, c->stream, c->dstatus, c->hstat);
-    HIPCHK(c, hipGetLastError());
-    c->status_pending = true;
     return NRG_OK;
 }

@@ -537,6 +537,7 @@ nrg_status nrg_ctx_create(nrg_ctx **out, int device) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_wait, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(&c->dstatus, sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->hstat, sizeof(int), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_values, hipEventDisableTiming);
     for (int k = 0; k < nrg_ctx::NSTAGE && e == hipSuccess; ++k)
         e = hipEventCreateWithFlags(&c->desc_ev[k], hipEventDisableTiming);
     if (e != hipSuccess) {
@@ -563,6 +564,8 @@ void nrg_ctx_destroy(nrg_ctx *c) {
     if (c->gpart) (void)hipFree(c->gpart);
     if (c->dstatus) (void)hipFree(c->dstatus);
     if (c->hstat) (void)hipHostFree(c->hstat);
+    if (c->hval) (void)hipHostFree(c->hval);
+    if (c->ev_values) (void)hipEventDestroy(c->ev_values);
     if (c->hpin) (void)hipHostFree(c->hpin);
     if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -574,6 +577,17 @@ void nrg_ctx_destroy(nrg_ctx *c) {
 }
 
 const char *nrg_last_error(const nrg_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+nrg_status nrg_diag_values(nrg_ctx *c, double *host, size_t count) {
+    if (!c->values_pending) return fail(c, NRG_ERR_ARG, "diag_values: no eigensolve is queued");
+    if (count != c->values_count) return fail(c, NRG_ERR_ARG, "diag_values: count differs from the batch's total dimension");
+    nrg_status s = wait_event(c, c->ev_values);
+    if (s) return s;
+    c->values_pending = false;
+    if (*c->hstat) return fail(c, NRG_ERR_HIP, "diag: a tridiagonalisation team did not assemble in time");
+    if (count) memcpy(host, c->hval, count * sizeof(double));
+    return NRG_OK;
+}
 
 nrg_status nrg_sync(nrg_ctx *c) {
     nrg_status s = wait_stream(c);

@@ -50,6 +50,13 @@ struct nrg_ctx {
     // a queued eigensolve's status word has not been read yet (wait_stream
     // reads it once the stream has drained)
     bool status_pending = false;
+    // eigenvalues of the last eigensolve, copied to pinned memory as soon as
+    // they are final (before the eigenvectors); ev_values follows that copy
+    double *hval = nullptr;
+    size_t hval_doubles = 0;
+    hipEvent_t ev_values = nullptr;
+    bool values_pending = false;
+    size_t values_count = 0;
     double *hpin = nullptr;
     size_t hpin_doubles = 0;
 };

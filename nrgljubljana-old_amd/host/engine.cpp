@@ -994,18 +994,35 @@ struct Engine {
                     deferred();
                     deferred = nullptr;
                 }
-                std::vector<nrg_strided> items;
-                size_t tot = 0;
-                for (auto &kv : it.subs) {
-                    items.push_back({kv.second.W, kv.second.d, 1});
-                    tot += kv.second.d;
-                }
-                std::vector<double> all(tot);
-                gpu.gather(items, all.data());
-                size_t pos = 0;
-                for (auto &kv : it.subs) {
-                    kv.second.value.assign(all.begin() + pos, all.begin() + pos + kv.second.d);
-                    pos += kv.second.d;
+                if (!opt.diag && !opt.plain) {
+                    // eigenvalues as soon as they are final (in task order),
+                    // while the eigenvectors are back-transformed
+                    size_t tot = 0;
+                    std::map<const double *, size_t> at;  // W -> position among the values
+                    for (auto &t : tasks) {
+                        at[t.W] = tot;
+                        tot += t.n;
+                    }
+                    std::vector<double> all(tot);
+                    gpu.check(nrg_diag_values(gpu.ctx, all.data(), tot), "nrg_diag_values");
+                    for (auto &kv : it.subs) {
+                        const size_t p = at.at(kv.second.W);
+                        kv.second.value.assign(all.begin() + p, all.begin() + p + kv.second.d);
+                    }
+                } else {
+                    std::vector<nrg_strided> items;
+                    size_t tot = 0;
+                    for (auto &kv : it.subs) {
+                        items.push_back({kv.second.W, kv.second.d, 1});
+                        tot += kv.second.d;
+                    }
+                    std::vector<double> all(tot);
+                    gpu.gather(items, all.data());
+                    size_t pos = 0;
+                    for (auto &kv : it.subs) {
+                        kv.second.value.assign(all.begin() + pos, all.begin() + pos + kv.second.d);
+                        pos += kv.second.d;
+                    }
                 }
             });
             // ---- ground state, truncation (src/nrg.cc:2881-2929)

@@ -13,7 +13,7 @@ PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(PKG, "lib", "libnrg_gpu.so")
 
 EXPORTS = ["nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_sync", "nrg_dev_alloc", "nrg_dev_free",
-           "nrg_h2d", "nrg_d2h", "nrg_d2h_strided", "nrg_zero", "nrg_gather_d2h", "nrg_gather_dev", "nrg_scatter_h2d", "nrg_mem_info", "nrg_host_alloc", "nrg_host_free", "nrg_assemble_batch", "nrg_diag_batch",
+           "nrg_h2d", "nrg_d2h", "nrg_d2h_strided", "nrg_zero", "nrg_gather_d2h", "nrg_gather_dev", "nrg_scatter_h2d", "nrg_mem_info", "nrg_host_alloc", "nrg_host_free", "nrg_assemble_batch", "nrg_diag_batch", "nrg_diag_values",
            "nrg_gemm_grouped", "nrg_spec_bin", "nrg_timing_enable", "nrg_timing_filter", "nrg_timing_reset",
            "nrg_timing_get"]
 
@@ -132,6 +132,14 @@ class Ctx:
             ptrs.append((A, W, n))
         self.check(self.lib.nrg_diag_batch(self.ctx, tasks, len(mats)), "diag_batch")
         return [(self.get(W, (n,)), self.get(A, (n, n))) for A, W, n in ptrs]
+
+    def diag_values(self, count):
+        """Eigenvalues of the last diag batch, concatenated in task order, as
+        soon as they are final (nrg_diag_values)."""
+        out = np.empty(count, dtype=np.float64)
+        self.check(self.lib.nrg_diag_values(self.ctx, out.ctypes.data_as(C.c_void_p), C.c_size_t(count)),
+                   "diag_values")
+        return out
 
     def gemm_grouped(self, outs, terms):
         o = (GemmOut * len(outs))(*outs)

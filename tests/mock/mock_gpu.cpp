@@ -143,9 +143,22 @@ static void jacobi(int n, double *A, double *W) {
     }
 }
 
+static std::vector<nrg_eig_task> g_last_diag;  // for nrg_diag_values
 nrg_status nrg_diag_batch(nrg_ctx *, const nrg_eig_task *t, int nt) {
     for (int i = 0; i < nt; ++i) jacobi(t[i].n, t[i].A, t[i].W);
+    g_last_diag.assign(t, t + nt);
     return NRG_OK;
+}
+
+nrg_status nrg_diag_values(nrg_ctx *, double *host, size_t count) {
+    size_t pos = 0;
+    for (const nrg_eig_task &t : g_last_diag) {
+        if (pos + t.n > count) return NRG_ERR_ARG;
+        for (int r = 0; r < t.n; ++r) host[pos + r] = t.W[r];
+        pos += t.n;
+    }
+    g_last_diag.clear();
+    return pos == count ? NRG_OK : NRG_ERR_ARG;
 }
 
 nrg_status nrg_gemm_grouped(nrg_ctx *, const nrg_gemm_out *outs, int no, const nrg_gemm_term *terms, int) {

@@ -116,6 +116,30 @@ def test_diag_batch_rejects_oversize(ctx):
     assert st != 0 and b"dimension" in ctx.lib.nrg_last_error(ctx.ctx)
 
 
+def test_diag_values_early_in_task_order(ctx):
+    # nrg_diag_values returns the batch's eigenvalues, concatenated in the
+    # order the tasks were passed (the eigensolver sorts them by size), the
+    # same bits as reading W back once everything has finished
+    capi = _capi()
+    rng = np.random.default_rng(12)
+    sizes = [5, 130, 1, 300, 64, 2, 129]
+    tasks = (capi.EigTask * len(sizes))()
+    ws = []
+    for t, n in zip(tasks, sizes):
+        m = rng.standard_normal((n, n))
+        t.A, t.W, t.n = ctx.put(m + m.T), ctx.alloc(n), n
+        ws.append((t.W, n))
+    assert ctx.lib.nrg_diag_batch(ctx.ctx, tasks, len(sizes)) == 0
+    early = ctx.diag_values(sum(sizes))
+    late = np.concatenate([ctx.get(w, (n,)) for w, n in ws])
+    np.testing.assert_array_equal(early, late)
+    out = np.empty(4)
+    # nothing queued any more, and a count that is not the batch's
+    assert ctx.lib.nrg_diag_values(ctx.ctx, out.ctypes.data_as(C.c_void_p), C.c_size_t(4)) != 0
+    assert ctx.lib.nrg_diag_batch(ctx.ctx, tasks, 2) == 0
+    assert ctx.lib.nrg_diag_values(ctx.ctx, out.ctypes.data_as(C.c_void_p), C.c_size_t(4)) != 0
+
+
 def test_diag_batch_many_small_with_shared_spectrum(ctx):
     # many blocks of one iteration: identical spectra in different blocks and
     # clusters inside blocks (SU(2) multiplets of neighbouring subspaces)
@@ -244,6 +268,52 @@ def test_assemble_batch(ctx):
     ref[:6, 6:] += -0.7 * F[:, :14]
     ref[6:, :6] += -0.7 * F[:, :14].T
     np.testing.assert_allclose(ctx.get(H, (d, d)), ref, rtol=0, atol=1e-15)
+
+
+def test_assemble_batch_bands(ctx):
+    # blocks of several sizes, each cut into three parts: diagonal vectors and
+    # shifts on every part, block terms (padded leading dimension) between
+    # every pair of parts, so that terms and their transposed halves cross the
+    # row bands the assembly is split into
+    capi = _capi()
+    rng = np.random.default_rng(11)
+
+    def term(kind, src, scale, r0, c0, rows, cols, ld):
+        t = capi.AsmTerm()
+        t.src, t.scale, t.kind, t.r0, t.c0, t.rows, t.cols, t.ld = src, scale, kind, r0, c0, rows, cols, ld
+        return t
+
+    blocks, terms, refs = [], [], []
+    for d in (1, 17, 100, 257):
+        H = ctx.alloc(d * d)
+        c = sorted(int(x) for x in rng.integers(0, d + 1, size=2))
+        parts = [(0, c[0]), (c[0], c[1]), (c[1], d)]
+        ref = np.zeros((d, d))
+        tb = len(terms)
+        for a0, a1 in parts:
+            if a1 > a0:
+                e = rng.standard_normal(a1 - a0)
+                idx = np.arange(a0, a1)
+                terms.append(term(0, ctx.put(e), 0.9, a0, 0, a1 - a0, 0, 0))
+                ref[idx, idx] += 0.9 * e
+                terms.append(term(2, None, -0.3, a0, 0, a1 - a0, 0, 0))
+                ref[idx, idx] += -0.3
+        for p_, q_ in ((0, 1), (0, 2), (1, 2)):
+            (r0, r1), (c0, c1) = parts[p_], parts[q_]
+            if r1 > r0 and c1 > c0:
+                ld = c1 - c0 + 3
+                F = rng.standard_normal((r1 - r0, ld))
+                sc = float(rng.standard_normal())
+                terms.append(term(1, ctx.put(F), sc, r0, c0, r1 - r0, c1 - c0, ld))
+                ref[r0:r1, c0:c1] += sc * F[:, :c1 - c0]
+                ref[c0:c1, r0:r1] += sc * F[:, :c1 - c0].T
+        b = capi.AsmBlock()
+        b.H, b.d, b.term_begin, b.term_count = H, d, tb, len(terms) - tb
+        blocks.append(b)
+        refs.append((H, d, ref))
+    ctx.assemble(blocks, terms)
+    for H, d, ref in refs:
+        np.testing.assert_allclose(ctx.get(H, (d, d)), ref, rtol=0, atol=1e-14)
 
 
 @pytest.mark.parametrize("mode", ["FDM", "DMNRG"])
