@@ -2290,17 +2290,10 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->dstatus, 1);
         HIPCHK(c, hipGetLastError());
         // the one-workgroup matrices run beside the teams on the side stream;
-        // they wait for nothing, so team workgroups they delay still start
-        if (!ones.empty()) {
-            HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
-            HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
-            {
-                LaunchTimer lt(K_EIG_TRI1, c->side, fl1, by1);
-                hipLaunchKernelGGL(k_eig_tri_one, dim3(ones.size()), dim3(TRI_NT), 0, c->side, d_mats,
-                                   (const int *)(db + o_ones), c->dstatus);
-            }
-            HIPCHK(c, hipEventRecord(c->ev_join, c->side));
-        }
+        // they wait for nothing, so team workgroups they delay still start.
+        // The teams are launched first (the fork point is recorded before
+        // them), so the side stream's set-up does not hold them back.
+        if (!ones.empty()) HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
         // grids <= co-resident capacity (one workgroup per CU at these sizes)
         // and nothing else runs on the stream, so every team assembles; the
         // barrier's bounded wait reports an error otherwise
@@ -2314,6 +2307,15 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             LaunchTimer lt(K_EIG_TRI, c->stream, flt, byt);
             hipLaunchKernelGGL(k_eig_tri_coop, dim3(tri_grid), dim3(TRI_NT), TRI_COOP_LDS, c->stream, d_mats, d_jobs,
                                d_tb, d_qj, c->cnt + creg, c->dstatus);
+        }
+        if (!ones.empty()) {
+            HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+            {
+                LaunchTimer lt(K_EIG_TRI1, c->side, fl1, by1);
+                hipLaunchKernelGGL(k_eig_tri_one, dim3(ones.size()), dim3(TRI_NT), 0, c->side, d_mats,
+                                   (const int *)(db + o_ones), c->dstatus);
+            }
+            HIPCHK(c, hipEventRecord(c->ev_join, c->side));
         }
         if (!ones.empty()) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     }
