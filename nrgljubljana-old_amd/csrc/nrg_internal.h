@@ -25,7 +25,7 @@ struct nrg_ctx {
     // staging ring for descriptor arrays: slot k (device buffer, pinned host
     // buffer) is reused once the event recorded when the next slot was taken
     // has completed, so the host runs up to NSTAGE launches ahead of the GPU
-    static constexpr int NSTAGE = 8;
+    static constexpr int NSTAGE = 32;
     void *desc[NSTAGE] = {};
     void *hdesc[NSTAGE] = {};
     size_t desc_bytes[NSTAGE] = {};
