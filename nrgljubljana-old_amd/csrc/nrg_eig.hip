@@ -917,6 +917,68 @@ __device__ inline void dc_bitonic_sort(double *key, int *idx, int K, int NT) {
 // Block-wide exclusive scans of one int per thread (sum, or max with
 // identity -1); every thread also receives the total.
 template <bool MAX>
+__device__ inline int dc_block_scan(int x, int *sred, int &total);
+
+// The eigenvalues of a merge are the two halves' outputs, each of them
+// ascending roots followed by ascending deflated values: a few ascending
+// runs.  When key[0, K) has at most DEFL_RUNS of them, this sorts it into
+// skey / idx in the order of dc_bitonic_sort ((key, index) ascending, a total
+// order, so the permutation is the same): the rank of element i is its offset
+// in its own run plus, for every other run, the number of that run's elements
+// before it (key <= x in earlier runs, key < x in later ones; binary
+// searches).  key and skey must differ; returns false, writing nothing, when
+// there are more runs.
+constexpr int DEFL_RUNS = 16;
+__device__ inline bool dc_run_sort(const double *key, double *skey, int *idx, int K, int *runs, int *sred) {
+    const int tid = threadIdx.x;
+    const int per = (K + DEFL_NT - 1) / DEFL_NT, q0 = min(K, tid * per), q1 = min(K, q0 + per);
+    int nb = 0;
+    for (int q = max(q0, 1); q < q1; ++q) nb += key[q] < key[q - 1];
+    int total;
+    int base = 1 + dc_block_scan<false>(nb, sred, total);
+    const int nr = total + 1;
+    if (nr > DEFL_RUNS) return false;
+    for (int q = max(q0, 1); q < q1; ++q)
+        if (key[q] < key[q - 1]) runs[base++] = q;
+    if (tid == 0) {
+        runs[0] = 0;
+        runs[nr] = K;
+    }
+    __syncthreads();
+    for (int i = tid; i < K; i += DEFL_NT) {
+        const double x = key[i];
+        int rank = 0;
+        for (int r = 0; r < nr; ++r) {
+            const int lo = runs[r], hi = runs[r + 1];
+            if (i >= lo && i < hi) {
+                rank += i - lo;
+                continue;
+            }
+            int l = lo, h = hi;
+            if (hi <= i) {
+                while (l < h) {
+                    const int m = (l + h) >> 1;
+                    if (key[m] <= x) l = m + 1;
+                    else h = m;
+                }
+            } else {
+                while (l < h) {
+                    const int m = (l + h) >> 1;
+                    if (key[m] < x) l = m + 1;
+                    else h = m;
+                }
+            }
+            rank += l - lo;
+        }
+        idx[rank] = i;
+    }
+    __syncthreads();
+    for (int q = tid; q < K; q += DEFL_NT) skey[q] = key[idx[q]];
+    __syncthreads();
+    return true;
+}
+
+template <bool MAX>
 __device__ inline int dc_block_scan(int x, int *sred, int &total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int inc = x;
@@ -999,12 +1061,22 @@ __global__ __launch_bounds__(DEFL_NT) void k_dc_deflate(const DcMerge *merges, n
     }
     int *kind = islot(ws, n, V_KIND) + a;
     double *rc = vslot(ws, n, V_RC) + a, *rsn = vslot(ws, n, V_RS) + a;
-    for (int i = tid; i < K; i += DEFL_NT) {
-        ds[i] = D[i];
-        perm[i] = i;
+    __shared__ int s_runs[DEFL_RUNS + 1];
+    // unsorted keys: an LDS copy in zs (written only after the sort)
+    const double *key = D;
+    if (IN_LDS) {
+        for (int i = tid; i < K; i += DEFL_NT) zs[i] = D[i];
+        __syncthreads();
+        key = zs;
     }
-    __syncthreads();
-    dc_bitonic_sort(ds, perm, K, DEFL_NT);
+    if (!dc_run_sort(key, ds, perm, K, s_runs, sred)) {
+        for (int i = tid; i < K; i += DEFL_NT) {
+            ds[i] = D[i];
+            perm[i] = i;
+        }
+        __syncthreads();
+        dc_bitonic_sort(ds, perm, K, DEFL_NT);
+    }
     double md = 0.0, mz = 0.0;
     for (int q = tid; q < K; q += DEFL_NT) {
         const int i = perm[q];
