@@ -395,16 +395,20 @@ constexpr int REG_ONE_N = 128;  // one workgroup: 16 waves x 8 rows, 2 column re
 constexpr int REG_MAX_N = 576;  // largest team variant: 9 column registers
 constexpr int REG_NT = 512;     // team workgroups: 8 waves, two per SIMD (one per SIMD exposes the
                                 // latency of each step's reduction chain: 12% slower)
-// rows per wavefront of the d <= 256 / 384 / 576 variants: few rows keep the
-// per-step row reductions short (more workgroups per team, up to 24 of the
-// 32 CUs of an XCD at d = 384 and 576); one row at d <= 256 leaves too few
-// CUs to the one-workgroup matrices on the side stream
-constexpr int REG_RPW[3] = {2, 2, 3};
+// Variants by column registers NC (d <= 64 NC): two rows per wavefront up to
+// d = 512 (teams of up to 32 workgroups, one XCD), three at d <= 576.  Few
+// rows keep the per-step row reductions short and a tight NC keeps the
+// per-step vector work small (d = 416 with NC = 9 and three rows: 3.8 us
+// per step; NC = 7 and two rows on 26 workgroups is the same work per lane
+// as d <= 448's share).  One row at d <= 256 leaves too few CUs to the
+// one-workgroup matrices on the side stream.
+constexpr int REG_TWO_ROWS_N = 512;
+__host__ __device__ constexpr int reg_rpw(int n) { return n <= REG_TWO_ROWS_N ? 2 : 3; }
 
 // team size of the register variant for dimension n (REG_ONE_N < n <=
-// REG_MAX_N): workgroups of 8 waves x REG_RPW rows
+// REG_MAX_N): workgroups of 8 waves x reg_rpw(n) rows
 __host__ __device__ inline int reg_team(int n) {
-    const int rows = (REG_NT / 64) * (n <= 256 ? REG_RPW[0] : n <= 384 ? REG_RPW[1] : REG_RPW[2]);
+    const int rows = (REG_NT / 64) * reg_rpw(n);
     return (n + rows - 1) / rows;
 }
 
@@ -691,10 +695,18 @@ __global__ __launch_bounds__(TRI_NT) void k_eig_tri_one(const EigMat *mats, cons
     tri_reg<2, 8, TRI_NW, false>(mt, 0, 1, nullptr, status, nullptr, lbuf, vsh);
 }
 
+// One workgroup on a matrix of the queue that follows the teams; not inlined,
+// so that its registers are allocated apart from the team variants'.
+template <int NC, int RPW, int NW>
+__device__ __noinline__ void tri_one_wg(const EigMat &mt, int *status, double *lbuf, double *vsh) {
+    tri_reg<NC, RPW, NW, false>(mt, 0, 1, nullptr, status, nullptr, lbuf, vsh);
+}
+
 // One persistent launch for the register-resident teams: workgroups of XCD
 // x draw tickets from queue x (a team stays on one XCD), as k_eig_tri_coop.
 __global__ __launch_bounds__(REG_NT) void k_eig_tri_reg(const EigMat *mats, const TriJob *jobs, const int *tbase,
-                                                        const int *qjob, int *counters, int *status) {
+                                                        const int *qjob, int *counters, int *status, const int *ones,
+                                                        int nones) {
     __shared__ int s_ticket, s_flag;
     __shared__ double lbuf[2 * 64 * (REG_MAX_N / 64)];
     __shared__ double vsh[2 * 64 * (REG_MAX_N / 64)];
@@ -718,9 +730,30 @@ __global__ __launch_bounds__(REG_NT) void k_eig_tri_reg(const EigMat *mats, cons
         const TriJob jb = jobs[lo];
         const EigMat mt = mats[jb.mat];
         const int rank = t0 + t - tbase[lo];
-        if (mt.n <= 256) tri_reg<4, REG_RPW[0], REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf, vsh);
-        else if (mt.n <= 384) tri_reg<6, REG_RPW[1], REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf, vsh);
-        else tri_reg<9, REG_RPW[2], REG_NT / 64, true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag, lbuf, vsh);
+        int *const bar = bars + BAR_STRIDE * lo;
+        constexpr int NW = REG_NT / 64;
+        if (mt.n <= 256) tri_reg<4, reg_rpw(256), NW, true>(mt, rank, jb.T, bar, status, &s_flag, lbuf, vsh);
+        else if (mt.n <= 320) tri_reg<5, reg_rpw(320), NW, true>(mt, rank, jb.T, bar, status, &s_flag, lbuf, vsh);
+        else if (mt.n <= 384) tri_reg<6, reg_rpw(384), NW, true>(mt, rank, jb.T, bar, status, &s_flag, lbuf, vsh);
+        else if (mt.n <= 448) tri_reg<7, reg_rpw(448), NW, true>(mt, rank, jb.T, bar, status, &s_flag, lbuf, vsh);
+        else if (mt.n <= 512) tri_reg<8, reg_rpw(512), NW, true>(mt, rank, jb.T, bar, status, &s_flag, lbuf, vsh);
+        else tri_reg<9, reg_rpw(576), NW, true>(mt, rank, jb.T, bar, status, &s_flag, lbuf, vsh);
+        __syncthreads();
+    }
+    // then the one-workgroup matrices (n <= REG_ONE_N, largest first) from a
+    // queue shared by all XCDs: every CU already holds a workgroup of this
+    // launch, so they need no CU of their own
+    int *oticket = counters + N_XCD;
+    for (;;) {
+        if (threadIdx.x == 0) s_ticket = atomicAdd(oticket, 1);
+        __syncthreads();
+        const int t = s_ticket;
+        __syncthreads();
+        if (t >= nones) break;
+        const EigMat mt = mats[ones[t]];
+        constexpr int NW = REG_NT / 64;
+        if (mt.n <= 64) tri_one_wg<1, 64 / NW, NW>(mt, status, lbuf, vsh);
+        else tri_one_wg<2, REG_ONE_N / NW, NW>(mt, status, lbuf, vsh);
         __syncthreads();
     }
 }
@@ -2361,26 +2394,29 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, c->stream, c->cnt, (int)ncnt);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->dstatus, 1);
         HIPCHK(c, hipGetLastError());
-        // the one-workgroup matrices run beside the teams on the side stream;
-        // they wait for nothing, so team workgroups they delay still start.
-        // The teams are launched first (the fork point is recorded before
-        // them), so the side stream's set-up does not hold them back.
-        if (!ones.empty()) HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+        // With register teams, the one-workgroup matrices are drained by that
+        // launch's workgroups without a team ticket (a separate launch would
+        // need free CUs, and its workgroups wait when the dispatcher places
+        // them where teams hold every CU).  Without teams they run in their
+        // own launch on the side stream, beside the streaming teams.
+        const bool ones_in_reg = !rjobs.empty();
+        const bool ones_side = !ones.empty() && !ones_in_reg;
+        if (ones_side) HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
         // grids <= co-resident capacity (one workgroup per CU at these sizes)
         // and nothing else runs on the stream, so every team assembles; the
         // barrier's bounded wait reports an error otherwise
         if (!rjobs.empty()) {
-            LaunchTimer lt(K_EIG_TRI_REG, c->stream, flr, byr);
+            LaunchTimer lt(K_EIG_TRI_REG, c->stream, flr + fl1, byr + by1);
             hipLaunchKernelGGL(k_eig_tri_reg, dim3(reg_grid), dim3(REG_NT), 0, c->stream, d_mats,
                                (const TriJob *)(db + o_rjobs), (const int *)(db + o_rtb), (const int *)(db + o_rqj),
-                               c->cnt, c->dstatus);
+                               c->cnt, c->dstatus, (const int *)(db + o_ones), (int)ones.size());
         }
         if (!jobs.empty()) {
             LaunchTimer lt(K_EIG_TRI, c->stream, flt, byt);
             hipLaunchKernelGGL(k_eig_tri_coop, dim3(tri_grid), dim3(TRI_NT), TRI_COOP_LDS, c->stream, d_mats, d_jobs,
                                d_tb, d_qj, c->cnt + creg, c->dstatus);
         }
-        if (!ones.empty()) {
+        if (ones_side) {
             HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
             {
                 LaunchTimer lt(K_EIG_TRI1, c->side, fl1, by1);
@@ -2388,8 +2424,8 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                                    (const int *)(db + o_ones), c->dstatus);
             }
             HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+            HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
         }
-        if (!ones.empty()) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
     }
     HIPCHK(c, hipGetLastError());
     // once the eigenvalues are final: the status word (set only by the
