@@ -2070,6 +2070,184 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     int nbig = 0;  // matrices with n >= 2 form the prefix
     while (nbig < ntasks && tasks[nbig].n >= 2) ++nbig;
 
+    // tridiagonalisation jobs: ticket ranges, team sizes
+    if (c->tri_capacity <= 0) {
+        int per_cu = 0, dev = 0, ncu = 0;
+        HIPCHK(c, hipGetDevice(&dev));
+        HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        HIPCHK(c, hipFuncSetAttribute((const void *)k_eig_tri_coop, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)TRI_COOP_LDS));
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_coop, TRI_NT, TRI_COOP_LDS));
+        c->tri_capacity = std::max(1, per_cu * ncu);
+        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_reg, REG_NT, 0));
+        // at most one team workgroup per CU: two would share its SIMDs
+        c->reg_capacity = std::max(1, std::min(per_cu, 1) * ncu);
+        HIPCHK(c, hipFuncSetAttribute((const void *)k_dc_small, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)sizeof(DcSmall)));
+    }
+    // Team sizes, then longest-processing-time assignment of the jobs to the
+    // XCD queues (work = estimated time x workgroups; a team stays on one XCD)
+    const int tri_grid = c->tri_capacity;
+    const int per_xcd = std::max(1, tri_grid / N_XCD);
+    const int reg_grid = c->reg_capacity, reg_per_xcd = std::max(1, reg_grid / N_XCD);
+    std::vector<int> teamT(ntasks, 1);
+    std::vector<double> cost(ntasks);
+    std::vector<char> global(ntasks, 0);
+    std::vector<char> regjob(ntasks, 0);
+    for (int i = 0; i < ntasks; ++i) {
+        const double n = tasks[i].n;
+        if (tasks[i].n <= REG_ONE_N) {  // one workgroup on the side stream, not queued
+            cost[i] = 0.5 * n;
+            continue;
+        }
+        // microseconds per step: a register-resident team pays its exchange;
+        // a streaming team a barrier plus its share of the matrix
+        if (tasks[i].n <= REG_MAX_N && reg_team(tasks[i].n) <= reg_per_xcd) {
+            teamT[i] = reg_team(tasks[i].n);
+            regjob[i] = 1;
+            cost[i] = 2.0 * n;
+            continue;
+        }
+        const int nsl = (tasks[i].n - 1 + TEAM_W - 1) / TEAM_W;
+        int T;
+        if (tasks[i].n > TEAM_GLOBAL_N) {
+            T = std::min({nsl, TEAM_GLOBAL_MAX, tri_grid});
+            global[i] = 1;
+        } else {
+            T = std::min({nsl, TEAM_MAX, per_xcd});
+        }
+        teamT[i] = std::max(T, 2);
+        cost[i] = n * (3.0 + 1.1e-4 * n * n / teamT[i]);
+        mats[i].pad = 1;  // reflectors to copy from S2
+    }
+    std::vector<int> order(ntasks);
+    for (int i = 0; i < ntasks; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<int> gq;
+    std::vector<std::vector<int>> queue(N_XCD), rqueue(N_XCD);
+    std::vector<double> load(N_XCD, 0.0), rload(N_XCD, 0.0);
+    for (int i : order)
+        if (global[i]) {
+            gq.push_back(i);
+            for (int x = 0; x < N_XCD; ++x) load[x] += cost[i] * teamT[i] / tri_grid;
+        }
+    std::vector<int> ones;  // one-workgroup matrices
+    for (int i : order) {
+        if (global[i]) continue;
+        if (tasks[i].n <= REG_ONE_N) {
+            ones.push_back(i);
+            continue;
+        }
+        std::vector<double> &ld = regjob[i] ? rload : load;
+        const int x = (int)(std::min_element(ld.begin(), ld.end()) - ld.begin());
+        (regjob[i] ? rqueue : queue)[x].push_back(i);
+        ld[x] += cost[i] * teamT[i] / (regjob[i] ? reg_per_xcd : per_xcd);
+    }
+    // job lists: [global queue, queue of XCD 0, ..., queue of XCD 7]
+    auto job_list = [&](const std::vector<int> &g, const std::vector<std::vector<int>> &q, std::vector<TriJob> &jobs,
+                        std::vector<int> &tbase, std::vector<int> &qjob) {
+        tbase.assign(1, 0);
+        qjob.assign(1, 0);
+        for (int i : g) {
+            jobs.push_back({i, teamT[i]});
+            tbase.push_back(tbase.back() + teamT[i]);
+        }
+        qjob.push_back((int)jobs.size());
+        for (int x = 0; x < N_XCD; ++x) {
+            for (int i : q[x]) {
+                jobs.push_back({i, teamT[i]});
+                tbase.push_back(tbase.back() + teamT[i]);
+            }
+            qjob.push_back((int)jobs.size());
+        }
+    };
+    std::vector<TriJob> jobs, rjobs;
+    std::vector<int> tbase, qjob, rtbase, rqjob;
+    job_list(gq, queue, jobs, tbase, qjob);
+    job_list({}, rqueue, rjobs, rtbase, rqjob);
+    // counters: tickets + one barrier per job, register teams first; barrier
+    // counters a cache line apart (teams polling neighbouring counters would
+    // contend for one line)
+    const size_t creg = N_XCD + 1 + (size_t)BAR_STRIDE * rjobs.size(),
+                 ncnt = creg + N_XCD + 1 + (size_t)BAR_STRIDE * jobs.size();
+    if (ncnt > c->cnt_ints) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->cnt) HIPCHK(c, hipFree(c->cnt));
+        c->cnt = nullptr;
+        c->cnt_ints = std::max<size_t>(ncnt * 2, 1024);
+        HIPCHK(c, hipMalloc(&c->cnt, c->cnt_ints * sizeof(int)));
+    }
+    // The tridiagonalisation is staged and launched before the rest of the
+    // plan is built, so the host builds that while it runs.  Its slot of the
+    // staging ring (the matrix descriptors, which the later stages read too)
+    // is next written by a copy queued on this stream after all of them.
+    HostArena ar1;
+    const size_t o_jobs = ar1.put(jobs), o_tb = ar1.put(tbase), o_qj = ar1.put(qjob), o_ones = ar1.put(ones);
+    const size_t o_rjobs = ar1.put(rjobs), o_rtb = ar1.put(rtbase), o_rqj = ar1.put(rqjob);
+    const size_t o_mats = ar1.put(mats);
+    void *d1;
+    {
+        const nrg_status s1 = stage(c, ar1.buf.data(), ar1.buf.size(), &d1);
+        if (s1) return s1;
+    }
+    char *db1 = (char *)d1;
+    const EigMat *d_mats = (const EigMat *)(db1 + o_mats);
+    const TriJob *d_jobs = (const TriJob *)(db1 + o_jobs);
+    const int *d_tb = (const int *)(db1 + o_tb);
+    const int *d_qj = (const int *)(db1 + o_qj);
+
+    // 1. tridiagonalisation: matrices up to REG_ONE_N one workgroup each,
+    //    then one persistent launch for the register-resident teams and one
+    //    for the streaming teams
+    {
+        double fl1 = 0, by1 = 0, flr = 0, byr = 0, flt = 0, byt = 0;
+        for (int i = 0; i < ntasks; ++i) {
+            const double n = tasks[i].n;
+            // algorithmic work: 4/3 n^3 flops; the trailing matrix is read once
+            // per column (sum_j 8 (n-j)^2 bytes)
+            double &fl = tasks[i].n <= REG_ONE_N ? fl1 : regjob[i] ? flr : flt;
+            double &by = tasks[i].n <= REG_ONE_N ? by1 : regjob[i] ? byr : byt;
+            fl += 4.0 / 3.0 * n * n * n;
+            by += 8.0 / 3.0 * n * n * n;
+        }
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, c->stream, c->cnt, (int)ncnt);
+        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->dstatus, 1);
+        HIPCHK(c, hipGetLastError());
+        // With register teams, the one-workgroup matrices are drained by that
+        // launch's workgroups without a team ticket (a separate launch would
+        // need free CUs, and its workgroups wait when the dispatcher places
+        // them where teams hold every CU).  Without teams they run in their
+        // own launch on the side stream, beside the streaming teams.
+        const bool ones_in_reg = !rjobs.empty();
+        const bool ones_side = !ones.empty() && !ones_in_reg;
+        if (ones_side) HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+        // grids <= co-resident capacity (one workgroup per CU at these sizes)
+        // and nothing else runs on the stream, so every team assembles; the
+        // barrier's bounded wait reports an error otherwise
+        if (!rjobs.empty()) {
+            LaunchTimer lt(K_EIG_TRI_REG, c->stream, flr + fl1, byr + by1);
+            hipLaunchKernelGGL(k_eig_tri_reg, dim3(reg_grid), dim3(REG_NT), 0, c->stream, d_mats,
+                               (const TriJob *)(db1 + o_rjobs), (const int *)(db1 + o_rtb), (const int *)(db1 + o_rqj),
+                               c->cnt, c->dstatus, (const int *)(db1 + o_ones), (int)ones.size());
+        }
+        if (!jobs.empty()) {
+            LaunchTimer lt(K_EIG_TRI, c->stream, flt, byt);
+            hipLaunchKernelGGL(k_eig_tri_coop, dim3(tri_grid), dim3(TRI_NT), TRI_COOP_LDS, c->stream, d_mats, d_jobs,
+                               d_tb, d_qj, c->cnt + creg, c->dstatus);
+        }
+        if (ones_side) {
+            HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+            {
+                LaunchTimer lt(K_EIG_TRI1, c->side, fl1, by1);
+                hipLaunchKernelGGL(k_eig_tri_one, dim3(ones.size()), dim3(TRI_NT), 0, c->side, d_mats,
+                                   (const int *)(db1 + o_ones), c->dstatus);
+            }
+            HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+            HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        }
+    }
+    HIPCHK(c, hipGetLastError());
+
     // divide-and-conquer tree, level by level over all matrices
     int maxlev = 0;
     for (int i = 0; i < nbig; ++i) maxlev = std::max(maxlev, dc_levels(tasks[i].n));
@@ -2239,118 +2417,9 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         }
         wylev[q] = L;
     }
-    // tridiagonalisation jobs: ticket ranges, team sizes
-    if (c->tri_capacity <= 0) {
-        int per_cu = 0, dev = 0, ncu = 0;
-        HIPCHK(c, hipGetDevice(&dev));
-        HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-        HIPCHK(c, hipFuncSetAttribute((const void *)k_eig_tri_coop, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)TRI_COOP_LDS));
-        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_coop, TRI_NT, TRI_COOP_LDS));
-        c->tri_capacity = std::max(1, per_cu * ncu);
-        HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_reg, REG_NT, 0));
-        // at most one team workgroup per CU: two would share its SIMDs
-        c->reg_capacity = std::max(1, std::min(per_cu, 1) * ncu);
-        HIPCHK(c, hipFuncSetAttribute((const void *)k_dc_small, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)sizeof(DcSmall)));
-    }
-    // Team sizes, then longest-processing-time assignment of the jobs to the
-    // XCD queues (work = estimated time x workgroups; a team stays on one XCD)
-    const int tri_grid = c->tri_capacity;
-    const int per_xcd = std::max(1, tri_grid / N_XCD);
-    const int reg_grid = c->reg_capacity, reg_per_xcd = std::max(1, reg_grid / N_XCD);
-    std::vector<int> teamT(ntasks, 1);
-    std::vector<double> cost(ntasks);
-    std::vector<char> global(ntasks, 0);
-    std::vector<char> regjob(ntasks, 0);
-    for (int i = 0; i < ntasks; ++i) {
-        const double n = tasks[i].n;
-        if (tasks[i].n <= REG_ONE_N) {  // one workgroup on the side stream, not queued
-            cost[i] = 0.5 * n;
-            continue;
-        }
-        // microseconds per step: a register-resident team pays its exchange;
-        // a streaming team a barrier plus its share of the matrix
-        if (tasks[i].n <= REG_MAX_N && reg_team(tasks[i].n) <= reg_per_xcd) {
-            teamT[i] = reg_team(tasks[i].n);
-            regjob[i] = 1;
-            cost[i] = 2.0 * n;
-            continue;
-        }
-        const int nsl = (tasks[i].n - 1 + TEAM_W - 1) / TEAM_W;
-        int T;
-        if (tasks[i].n > TEAM_GLOBAL_N) {
-            T = std::min({nsl, TEAM_GLOBAL_MAX, tri_grid});
-            global[i] = 1;
-        } else {
-            T = std::min({nsl, TEAM_MAX, per_xcd});
-        }
-        teamT[i] = std::max(T, 2);
-        cost[i] = n * (3.0 + 1.1e-4 * n * n / teamT[i]);
-        mats[i].pad = 1;  // reflectors to copy from S2
-    }
-    std::vector<int> order(ntasks);
-    for (int i = 0; i < ntasks; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::vector<int> gq;
-    std::vector<std::vector<int>> queue(N_XCD), rqueue(N_XCD);
-    std::vector<double> load(N_XCD, 0.0), rload(N_XCD, 0.0);
-    for (int i : order)
-        if (global[i]) {
-            gq.push_back(i);
-            for (int x = 0; x < N_XCD; ++x) load[x] += cost[i] * teamT[i] / tri_grid;
-        }
-    std::vector<int> ones;  // one-workgroup matrices
-    for (int i : order) {
-        if (global[i]) continue;
-        if (tasks[i].n <= REG_ONE_N) {
-            ones.push_back(i);
-            continue;
-        }
-        std::vector<double> &ld = regjob[i] ? rload : load;
-        const int x = (int)(std::min_element(ld.begin(), ld.end()) - ld.begin());
-        (regjob[i] ? rqueue : queue)[x].push_back(i);
-        ld[x] += cost[i] * teamT[i] / (regjob[i] ? reg_per_xcd : per_xcd);
-    }
-    // job lists: [global queue, queue of XCD 0, ..., queue of XCD 7]
-    auto job_list = [&](const std::vector<int> &g, const std::vector<std::vector<int>> &q, std::vector<TriJob> &jobs,
-                        std::vector<int> &tbase, std::vector<int> &qjob) {
-        tbase.assign(1, 0);
-        qjob.assign(1, 0);
-        for (int i : g) {
-            jobs.push_back({i, teamT[i]});
-            tbase.push_back(tbase.back() + teamT[i]);
-        }
-        qjob.push_back((int)jobs.size());
-        for (int x = 0; x < N_XCD; ++x) {
-            for (int i : q[x]) {
-                jobs.push_back({i, teamT[i]});
-                tbase.push_back(tbase.back() + teamT[i]);
-            }
-            qjob.push_back((int)jobs.size());
-        }
-    };
-    std::vector<TriJob> jobs, rjobs;
-    std::vector<int> tbase, qjob, rtbase, rqjob;
-    job_list(gq, queue, jobs, tbase, qjob);
-    job_list({}, rqueue, rjobs, rtbase, rqjob);
-    // counters: tickets + one barrier per job, register teams first; barrier
-    // counters a cache line apart (teams polling neighbouring counters would
-    // contend for one line)
-    const size_t creg = N_XCD + 1 + (size_t)BAR_STRIDE * rjobs.size(),
-                 ncnt = creg + N_XCD + 1 + (size_t)BAR_STRIDE * jobs.size();
-    if (ncnt > c->cnt_ints) {
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        if (c->cnt) HIPCHK(c, hipFree(c->cnt));
-        c->cnt = nullptr;
-        c->cnt_ints = std::max<size_t>(ncnt * 2, 1024);
-        HIPCHK(c, hipMalloc(&c->cnt, c->cnt_ints * sizeof(int)));
-    }
     HostArena ar;
-    const size_t o_jobs = ar.put(jobs), o_tb = ar.put(tbase), o_qj = ar.put(qjob), o_ones = ar.put(ones);
-    const size_t o_rjobs = ar.put(rjobs), o_rtb = ar.put(rtbase), o_rqj = ar.put(rqjob);
     const size_t o_small = ar.put(small), o_voff = ar.put(voff);
-    const size_t o_mats = ar.put(mats), o_merges = ar.put(merges), o_sch = ar.put(schunks), o_uch = ar.put(uchunks),
+    const size_t o_merges = ar.put(merges), o_sch = ar.put(schunks), o_uch = ar.put(uchunks),
                  o_xch = ar.put(gathers), o_cpy = ar.put(copies), o_gouts = ar.put(gouts), o_gterms = ar.put(gterms), o_gitems = ar.put(gitems), o_gred = ar.put(gred), o_rk = ar.put(rk),
                  o_cp = ar.put(cp);
     size_t o_bt[kBtVariants];
@@ -2365,10 +2434,6 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     nrg_status s = stage(c, ar.buf.data(), ar.buf.size(), &d);
     if (s) return s;
     char *db = (char *)d;
-    const EigMat *d_mats = (const EigMat *)(db + o_mats);
-    const TriJob *d_jobs = (const TriJob *)(db + o_jobs);
-    const int *d_tb = (const int *)(db + o_tb);
-    const int *d_qj = (const int *)(db + o_qj);
     const DcMerge *d_merges = (const DcMerge *)(db + o_merges);
     const DcChunk *d_sch = (const DcChunk *)(db + o_sch), *d_uch = (const DcChunk *)(db + o_uch),
                   *d_cpy = (const DcChunk *)(db + o_cpy);
@@ -2377,56 +2442,6 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     nrg_gemm_term *d_gterms = (nrg_gemm_term *)(db + o_gterms);
     const GemmItem *d_gitems = (const GemmItem *)(db + o_gitems), *d_gred = (const GemmItem *)(db + o_gred);
 
-    // 1. tridiagonalisation: matrices up to REG_ONE_N one workgroup each,
-    //    then one persistent launch for the register-resident teams and one
-    //    for the streaming teams
-    {
-        double fl1 = 0, by1 = 0, flr = 0, byr = 0, flt = 0, byt = 0;
-        for (int i = 0; i < ntasks; ++i) {
-            const double n = tasks[i].n;
-            // algorithmic work: 4/3 n^3 flops; the trailing matrix is read once
-            // per column (sum_j 8 (n-j)^2 bytes)
-            double &fl = tasks[i].n <= REG_ONE_N ? fl1 : regjob[i] ? flr : flt;
-            double &by = tasks[i].n <= REG_ONE_N ? by1 : regjob[i] ? byr : byt;
-            fl += 4.0 / 3.0 * n * n * n;
-            by += 8.0 / 3.0 * n * n * n;
-        }
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, c->stream, c->cnt, (int)ncnt);
-        hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->dstatus, 1);
-        HIPCHK(c, hipGetLastError());
-        // With register teams, the one-workgroup matrices are drained by that
-        // launch's workgroups without a team ticket (a separate launch would
-        // need free CUs, and its workgroups wait when the dispatcher places
-        // them where teams hold every CU).  Without teams they run in their
-        // own launch on the side stream, beside the streaming teams.
-        const bool ones_in_reg = !rjobs.empty();
-        const bool ones_side = !ones.empty() && !ones_in_reg;
-        if (ones_side) HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
-        // grids <= co-resident capacity (one workgroup per CU at these sizes)
-        // and nothing else runs on the stream, so every team assembles; the
-        // barrier's bounded wait reports an error otherwise
-        if (!rjobs.empty()) {
-            LaunchTimer lt(K_EIG_TRI_REG, c->stream, flr + fl1, byr + by1);
-            hipLaunchKernelGGL(k_eig_tri_reg, dim3(reg_grid), dim3(REG_NT), 0, c->stream, d_mats,
-                               (const TriJob *)(db + o_rjobs), (const int *)(db + o_rtb), (const int *)(db + o_rqj),
-                               c->cnt, c->dstatus, (const int *)(db + o_ones), (int)ones.size());
-        }
-        if (!jobs.empty()) {
-            LaunchTimer lt(K_EIG_TRI, c->stream, flt, byt);
-            hipLaunchKernelGGL(k_eig_tri_coop, dim3(tri_grid), dim3(TRI_NT), TRI_COOP_LDS, c->stream, d_mats, d_jobs,
-                               d_tb, d_qj, c->cnt + creg, c->dstatus);
-        }
-        if (ones_side) {
-            HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
-            {
-                LaunchTimer lt(K_EIG_TRI1, c->side, fl1, by1);
-                hipLaunchKernelGGL(k_eig_tri_one, dim3(ones.size()), dim3(TRI_NT), 0, c->side, d_mats,
-                                   (const int *)(db + o_ones), c->dstatus);
-            }
-            HIPCHK(c, hipEventRecord(c->ev_join, c->side));
-            HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
-        }
-    }
     HIPCHK(c, hipGetLastError());
     // once the eigenvalues are final: the status word (set only by the
     // tridiagonalisation teams) and the eigenvalues go to pinned memory, and
