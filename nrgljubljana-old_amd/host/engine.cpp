@@ -1374,12 +1374,20 @@ struct Engine {
             const RhoMap *rm = nullptr;
             if (fdm) rm = last ? nullptr : &rhofdm[N];
             else rm = last ? &rho_init : &rhos[N];
-            for (auto &ki : it.subs)
-                for (auto &kj : it.subs) {
-                    const Invar Ii = ki.first, Ij = kj.first;
+            // the nonzero blocks of op1, in the order of a double loop over the
+            // subspaces (Ii outer, Ij inner), which fixes the order of the pairs
+            // (probing every pair of subspaces cost ~0.1 ms per shell)
+            std::vector<std::pair<Invar, Invar>> keys;  // (Ii, Ij)
+            keys.reserve(op1.size());
+            for (auto &blk : op1) keys.push_back({blk.first.second, blk.first.first});
+            std::sort(keys.begin(), keys.end());
+            for (const auto &key : keys) {
+                    const Invar Ii = key.first, Ij = key.second;
+                    const auto ki = it.subs.find(Ii), kj = it.subs.find(Ij);
+                    if (ki == it.subs.end() || kj == it.subs.end()) continue;
                     const Twoinvar II{Ij, Ii};
                     auto a = op1.find(II), b = op2.find(II);
-                    if (a == op1.end() || b == op2.end()) continue;
+                    if (b == op2.end()) continue;
                     if (sp.spin != 0 && Ij.second - Ii.second != sp.spin) continue;
                     const DevMat &o1 = a->second, &o2 = b->second;
                     nrg_spec_pair q{};
@@ -1389,13 +1397,13 @@ struct Engine {
                     q.ld2 = o2.ld;
                     q.sf = T.specdens_factor(Ii, Ij);
                     if (fdm) {
-                        q.alli = ki.second.d;
-                        q.allj = kj.second.d;
+                        q.alli = ki->second.d;
+                        q.allj = kj->second.d;
                         if (o1.rows != q.allj || o1.cols != q.alli) die("FDM: operator block has the wrong size");
                         q.reti = last ? 0 : rm->at(Ii).dim;
                         q.retj = last ? 0 : rm->at(Ij).dim;
-                        q.Ei = ki.second.Eabs;
-                        q.Ej = kj.second.Eabs;
+                        q.Ei = ki->second.Eabs;
+                        q.Ej = kj->second.Eabs;
                         if (q.allj > 0 && q.reti > 0) {
                             double *M1 = tmp.alloc((size_t)q.allj * q.reti);
                             gb.out(M1, q.allj, q.reti, q.reti);
@@ -1416,8 +1424,8 @@ struct Engine {
                         q.allj = r1.dim;
                         q.reti = q.alli;
                         q.retj = q.allj;
-                        q.Ei = ki.second.W;
-                        q.Ej = kj.second.W;
+                        q.Ei = ki->second.W;
+                        q.Ej = kj->second.W;
                         if (q.alli == 0 || q.allj == 0) continue;
                         double *M1 = tmp.alloc((size_t)q.allj * q.alli);
                         double *M2 = tmp.alloc((size_t)q.allj * q.alli);
