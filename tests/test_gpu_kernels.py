@@ -47,10 +47,12 @@ def _check_eig(A, W, U, tol=1e-12):
 
 def test_diag_batch_random_sizes(ctx):
     rng = np.random.default_rng(1)
-    # boundaries of the tridiagonalisation variants: one workgroup up to 128,
-    # register teams of 4, 6 and 9 column registers up to 256 / 384 / 576,
-    # streaming teams above
-    sizes = [1, 2, 3, 4, 7, 16, 33, 64, 65, 127, 128, 129, 256, 257, 300, 384, 385, 511, 576, 577]
+    # boundaries of the tridiagonalisation variants: one workgroup up to 128
+    # (here drained by the team launch), register teams of 4 to 9 column
+    # registers (one per 64 columns; two rows per wavefront up to 512, three
+    # above), streaming teams above 576
+    sizes = [1, 2, 3, 4, 7, 16, 33, 64, 65, 127, 128, 129, 256, 257, 300, 320, 321, 384, 385, 448, 449, 511, 512,
+             513, 576, 577]
     mats = [_sym(rng, n) for n in sizes]
     out = ctx.diag_batch(mats)
     for A, (W, U) in zip(mats, out):
@@ -92,6 +94,16 @@ def test_diag_batch_large(ctx):
     # the divide and conquer above the LDS size of the deflation kernel
     rng = np.random.default_rng(3)
     mats = [_sym(rng, 700), _sym(rng, 1100), _sym(rng, 1345), _sym(rng, 2100)]
+    out = ctx.diag_batch(mats)
+    for A, (W, U) in zip(mats, out):
+        _check_eig(A, W, U, tol=2e-13)
+
+
+def test_diag_batch_streaming_teams_with_small_blocks(ctx):
+    # no register team: the one-workgroup matrices run in their own launch
+    # beside the streaming teams
+    rng = np.random.default_rng(8)
+    mats = [_sym(rng, n) for n in (700, 1, 50, 100, 128)]
     out = ctx.diag_batch(mats)
     for A, (W, U) in zip(mats, out):
         _check_eig(A, W, U, tol=2e-13)
