@@ -1156,6 +1156,13 @@ __global__ __launch_bounds__(DEFL_NT) void k_dc_deflate(const DcMerge *merges, n
     int cbase = dc_block_scan<false>(ncand_local, sred, ncand);
     for (int q = q0; q < q1; ++q)
         if (kind[q] & KIND_CAND) cand[cbase++] = q;
+    // the next large-z pole after each large-z pole (pos is free until the
+    // positions below): the walk leaves the poles after its current one as
+    // they are, so these successors stay valid while it runs
+    for (int q = q0; q < q1; ++q) pos[q] = -1;
+    __syncthreads();
+    for (int q = q0; q < q1; ++q)
+        if (!(rho * fabs(zs[q]) <= tol) && prev[q] >= 0) pos[prev[q]] = q;
     __syncthreads();
     if (tid == 0) {
         int ci = 0, chain = -1, nrot = 0;
@@ -1185,9 +1192,8 @@ __global__ __launch_bounds__(DEFL_NT) void k_dc_deflate(const DcMerge *merges, n
                 ds[p] = tt;
                 kind[p] |= KIND_SRC;
                 // the next large-z pole is tested against the rotated values
-                int nx = q + 1;
-                while (nx < K && rho * fabs(zs[nx]) <= tol) ++nx;
-                if (nx < K) chain = nx;
+                const int nx = pos[q];
+                if (nx >= 0) chain = nx;
             }
             kind[q] = k;
         }
