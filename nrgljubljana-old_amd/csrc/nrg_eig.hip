@@ -177,6 +177,8 @@ constexpr int BAR_STRIDE = 32;                    // ints between team barrier c
 
 struct TriJob {
     int mat, T;
+    int blocked;  // 1: blocked reduction (tri_blk), 0: streaming rows (tri_team)
+    int pad;
 };
 
 // (global address space, so that they are global_* rather than flat_*
@@ -192,7 +194,8 @@ __device__ inline double ld_dev(const double *p) {
 
 // Team barrier with a bounded wait: returns false (and flags the status) if
 // the team did not arrive, so that every workgroup still terminates.
-__device__ inline bool team_barrier(int *cnt, int target, int *status, int *s_flag) {
+__device__ inline bool team_barrier(int *cnt, int target, int *status, int *s_flag,
+                                    long long limit = TEAM_SPIN_LIMIT) {
     __builtin_amdgcn_s_waitcnt(0);  // this wave's device-scope stores have completed
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -201,7 +204,7 @@ __device__ inline bool team_barrier(int *cnt, int target, int *status, int *s_fl
         int ok = 1;
         while (__hip_atomic_load(cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(1);
-            if (++spins > TEAM_SPIN_LIMIT || __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+            if (++spins > limit || __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                 ok = 0;
                 atomicCAS(status, 0, NRG_ERR_HIP);
                 break;
@@ -758,8 +761,567 @@ __global__ __launch_bounds__(REG_NT) void k_eig_tri_reg(const EigMat *mats, cons
     }
 }
 
-__global__ __launch_bounds__(TRI_NT) void k_eig_tri_coop(const EigMat *mats, const TriJob *jobs, const int *tbase,
-                                                         const int *qjob, int *counters, int *status) {
+// ---------------------------------------------------------------------------
+// 1d. blocked tridiagonalisation for large matrices (the dsytrd / dlatrd
+//     scheme).  Panels of BLK_NB columns: inside a panel the trailing matrix
+//     keeps its value at the panel start, and column j and the product A v_j
+//     are corrected by the panel's earlier reflectors (A - V W^T - W V^T, with
+//     W kept as p - K v: p = tau A v, K = tau (p.v) / 2 per column); the panel
+//     ends with one rank-2 BLK_NB update of the trailing matrix (SYR2K) on
+//     FP64 MFMA.
+//     The product reads only the upper triangle of the row-major storage
+//     (c >= r: the lower triangle of the symmetric matrix read column-major).
+//     An off-diagonal 64 x 64 tile (R < C) gives both A x (row part, rows of
+//     R) and A^T x (column part, columns of C); diagonal tiles are kept whole
+//     (the SYR2K updates both of their halves) and give the row part alone.
+//     This halves the HBM traffic of the product, which bounds the whole
+//     reduction (sum_j 4 (n-j)^2 bytes, (4/3) n^3 per matrix).
+//     Ownership: 64-column block B (columns 64B .. 64B+63) belongs to team
+//     rank blk_owner(B) for the whole reduction; only that rank touches A's
+//     columns of B and rows B of the panel buffers with plain loads and
+//     stores (tile (R, C) is processed by owner(C)).  What crosses workgroups
+//     (column j, partial sums, the panel's V and p, the tiles' partial
+//     products) is written with device-scope stores and read by other ranks
+//     with device-scope loads.  Per column:
+//       P1  owners: column j corrected by the panel, its tail stored as the
+//           reflector, per-block partial sums |a|^2, W^T a, V^T a  -> barrier
+//       P2  all ranks: block sums in block order and the reflector (the same
+//           bits everywhere); owners: partial products of the tiles -> barrier
+//       P3  owners: p = tau (A v - V W^T v - W V^T v) from the partial
+//           products in a fixed order, per-block p.v                -> barrier
+//     Every sum runs per 64-block in an order fixed by (n, j) alone, so the
+//     result does not depend on the team size.
+//     Output as tri_team: reflector j in row j of A (columns j+1.., first
+//     element included), tau, diagonal and off-diagonal of T.
+// ---------------------------------------------------------------------------
+constexpr int BLK_NB = 32;   // panel width
+constexpr int BLK_TWS = 65;  // row stride of a wavefront's transpose buffer (16 rows of 64)
+constexpr int BLK_TW = 16 * BLK_TWS;
+constexpr int BLK_P1W = 1 + 2 * BLK_NB;          // partial sums per block: |a|^2, W^T a, V^T a
+constexpr long long BLK_SPIN_LIMIT = 1LL << 30;  // a member may first finish a long job of its own
+// dynamic LDS (doubles): 16 transpose buffers, 16 vector buffers, coefficient arrays
+constexpr int BLK_LDS_TW = 0, BLK_LDS_XS = BLK_LDS_TW + TRI_NW * BLK_TW, BLK_LDS_CF = BLK_LDS_XS + TRI_NW * 64;
+enum BlkCoef { CF_GAM = 0, CF_BET, CF_K, CF_UW, CF_UV, CF_WJ, CF_VJ1, CF_WJ1, CF_SC, CF_COUNT };
+constexpr size_t BLK_LDS = sizeof(double) * (BLK_LDS_CF + 64 * CF_COUNT);
+static_assert(BLK_LDS <= TRI_COOP_LDS, "the blocked reduction fits the team launch's LDS");
+static_assert(2 * 2 * BLK_NB * 64 <= TRI_NW * BLK_TW, "SYR2K operands fit in the transpose buffers");
+static_assert(16 * 16 * 64 <= TRI_NW * BLK_TW, "P3 partial sums fit in the transpose buffers");
+static_assert(BLK_NB <= 64 && 1 + 2 * BLK_NB <= 16 * 64, "coefficient arrays");
+
+// S1 layout of a blocked reduction (doubles from a 512-byte aligned start):
+// V and p of the panel (BLK_NB x npad each), the column, partial sums (P1,
+// P3), row and column partial products of the tiles (nt x nt x 64 each).
+__host__ __device__ inline size_t blk_ws_doubles(int n) {
+    const size_t npad = ((size_t)n + 63) & ~(size_t)63, nt = npad / 64;
+    return 64 + 2 * BLK_NB * npad + npad + nt * (BLK_P1W + 1) + 2 * nt * nt * 64;
+}
+__host__ __device__ inline int blk_owner(int B, int T) {
+    const int p = B % (2 * T);
+    return p < T ? p : 2 * T - 1 - p;  // snake: pairs of blocks balance the shrinking tiles
+}
+typedef double blk_d4 __attribute__((ext_vector_type(4)));
+
+// LDS written by some lanes of a wavefront and read by others: the wave
+// executes its LDS operations in order; this keeps the compiler from moving
+// them across each other.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// v_j at index i (zero outside [jj, n); v[jj] = v0, the tail from the column)
+__device__ __forceinline__ double blk_x(const double *av, int i, int jj, int n, double v0) {
+    return i == jj ? v0 : (i > jj && i < n) ? ld_dev(av + i) : 0.0;
+}
+
+// Partial products of tile (R, C) with x = v_j: row part sum_c A[r][c] x[c]
+// -> Prow[R][C], column part sum_r A[r][c] x[r] -> Pcol[C][R] (off-diagonal
+// tiles only).  One wavefront, lane = column; rows in chunks of 16, which
+// pass through the wave's LDS buffer for the row part (lane = row x quarter).
+// Edge tiles (rows or columns beyond n) load at clamped addresses and zero
+// the values outside the matrix.
+template <bool EDGE>
+__device__ __forceinline__ void blk_chunk_load(const gdouble *A, int cc, int rb, int n, bool cok, double (&a)[16]) {
+    // a uniform row base and 32-bit lane offsets (scalar-base addressing).
+    // Edge chunks clamp both the base row and every row to n - 1, so that
+    // the offset from the base is never negative (a chunk may start below
+    // the matrix's last row).
+    const int rbase = EDGE ? min(rb, n - 1) : rb;
+    const gdouble *rowb = A + (size_t)rbase * n;
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) a[rr] = rowb[(uint32_t)((EDGE ? min(rb + rr, n - 1) - rbase : rr) * n + cc)];
+    if (EDGE) {
+#pragma unroll
+        for (int rr = 0; rr < 16; ++rr) a[rr] = (cok && rb + rr < n) ? a[rr] : 0.0;
+    }
+}
+
+__device__ __forceinline__ void blk_chunk_use(const double (&a)[16], int q, double xr, double &colacc, double *Tw,
+                                              const double *xs, int lane, double *prow) {
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) {
+        colacc = fma(a[rr], readlane_d(xr, 16 * q + rr), colacc);
+        Tw[rr * BLK_TWS + lane] = a[rr];
+    }
+    wave_lds_sync();
+    const int r2 = lane & 15, g = lane >> 4;
+    double s = 0.0;
+#pragma unroll
+    for (int cc = 0; cc < 16; ++cc) s = fma(Tw[r2 * BLK_TWS + 16 * g + cc], xs[16 * g + cc], s);
+    s += __shfl_xor(s, 16, 64);
+    s += __shfl_xor(s, 32, 64);
+    if (lane < 16) st_dev(prow + 16 * q + lane, s);
+    wave_lds_sync();
+}
+
+template <bool EDGE>
+__device__ __forceinline__ void blk_tile_rows(const double *__restrict__ A, int n, int R, int c, bool cok, double xr,
+                                              double &colacc, double *Tw, const double *xs, int lane, double *prow) {
+    const gdouble *p = (const gdouble *)A;
+    const int cc = cok ? c : n - 1;
+    const int r0 = 64 * R;
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q) {
+        double a[16];
+        blk_chunk_load<EDGE>(p, cc, r0 + 16 * q, n, cok, a);
+        blk_chunk_use(a, q, xr, colacc, Tw, xs, lane, prow);
+    }
+}
+
+__device__ __forceinline__ void blk_tile(const double *__restrict__ A, int n, int nt, int R, int C, int jj, double v0,
+                                         const double *av, double *Prow, double *Pcol, double *Tw, double *xs,
+                                         int lane) {
+    const int c = 64 * C + lane;
+    const double xc = blk_x(av, c, jj, n, v0);
+    const double xr = blk_x(av, 64 * R + lane, jj, n, v0);
+    xs[lane] = xc;
+    double colacc = 0.0;
+    const bool cok = c < n;
+    double *prow = Prow + ((size_t)R * nt + C) * 64;
+    if (64 * C + 63 < n && 64 * R + 63 < n) blk_tile_rows<false>(A, n, R, c, cok, xr, colacc, Tw, xs, lane, prow);
+    else blk_tile_rows<true>(A, n, R, c, cok, xr, colacc, Tw, xs, lane, prow);
+    if (R != C) st_dev(Pcol + ((size_t)C * nt + R) * 64 + lane, colacc);
+}
+
+// The workgroup's tiles of the product (tile columns it owns, rows R0..C),
+// flattened over its wavefronts.  Not inlined: the tile loop gets registers
+// of its own.
+template <class P>
+__device__ __forceinline__ P *uni_ptr(P *p) {
+    const uint64_t u = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)u), hi = __builtin_amdgcn_readfirstlane((uint32_t)(u >> 32));
+    return (P *)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__device__ __noinline__ void blk_symv(const double *A_, int n_, int nt_, int R0_, int rank_, int T_, int jj_, double v0,
+                                      const double *av_, double *Prow_, double *Pcol_) {
+    extern __shared__ double lds[];
+    // arguments arrive in vector registers; they are uniform, and saying so
+    // keeps tile indices and addresses scalar.  So is the wavefront index.
+    // The LDS buffers are taken from the LDS array here (a pointer argument
+    // would lose its address space and every access would be a flat one).
+    const double *A = uni_ptr(A_), *av = uni_ptr(av_);
+    double *Prow = uni_ptr(Prow_), *Pcol = uni_ptr(Pcol_);
+    const int n = uni(n_), nt = uni(nt_), R0 = uni(R0_), rank = uni(rank_), T = uni(T_), jj = uni(jj_);
+    const int w = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    double *const Tw = lds + BLK_LDS_TW + w * BLK_TW;
+    double *const xs = lds + BLK_LDS_XS + w * 64;
+    int f = 0;
+    for (int C = R0; C < nt; ++C) {
+        if (blk_owner(C, T) != rank) continue;
+        const int cnt = C - R0 + 1;
+        for (int i = ((w - f) % TRI_NW + TRI_NW) % TRI_NW; i < cnt; i += TRI_NW)
+            blk_tile(A, n, nt, R0 + i, C, jj, v0, av, Prow, Pcol, Tw, xs, lane);
+        f += cnt;
+    }
+}
+
+// SYR2K of the finished panel on tile (R, C): A[r][c] -= sum_k V[r][k] W[c][k]
+// + W[r][k] V[c][k] for r, c >= j1 (both halves of a diagonal tile).  The
+// whole workgroup: the operands go to LDS (Ls = -[V W] at rows R, k-major;
+// Rs = [W V] at columns C, staged when C changes), then wavefront w computes
+// the 16 x 16 block (w / 4, w % 4) with v_mfma_f64_16x16x4f64 (A[i][k] in
+// lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16 (i % 4),
+// element i / 4: measured on gfx950), the old tile as the accumulator.
+__device__ __forceinline__ void blk_syr2k_tile(double *__restrict__ A, int n, int npad, int R, int C, int j1,
+                                               const double *Vb, const double *Wb, bool loadR) {
+    extern __shared__ double lds[];
+    double *const Ls = lds + BLK_LDS_TW, *const Rs = Ls + 2 * BLK_NB * 64;
+    const double *const Kc = lds + BLK_LDS_CF + 64 * CF_K;
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 2 * BLK_NB * 64 / TRI_NT; ++q) {
+        const int e = tid + q * TRI_NT, k = e >> 6, i = e & 63;
+        const int kk = k < BLK_NB ? k : k - BLK_NB;
+        const int r = 64 * R + i, cidx = 64 * C + i;
+        const double vr = ld_dev(Vb + (size_t)kk * npad + r), pr = ld_dev(Wb + (size_t)kk * npad + r);
+        double vc = 0.0, pc = 0.0;
+        if (loadR) {
+            vc = ld_dev(Vb + (size_t)kk * npad + cidx);
+            pc = ld_dev(Wb + (size_t)kk * npad + cidx);
+        }
+        Ls[e] = -(k < BLK_NB ? vr : pr - Kc[kk] * vr);
+        if (loadR) Rs[e] = k < BLK_NB ? pc - Kc[kk] * vc : vc;
+    }
+    __syncthreads();
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int bi = w >> 2, bj = w & 3;
+    const int col = 64 * C + 16 * bj + (lane & 15);
+    const int row0 = 64 * R + 16 * bi + (lane >> 4);  // element v: row row0 + 4 v
+    blk_d4 acc;
+    gdouble *Ag = (gdouble *)A;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const double x = Ag[(size_t)min(row0 + 4 * v, n - 1) * n + min(col, n - 1)];
+        acc[v] = (row0 + 4 * v < n && col < n) ? x : 0.0;
+    }
+#pragma unroll
+    for (int s = 0; s < 2 * BLK_NB / 4; ++s) {
+        const int k = 4 * s + (lane >> 4);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ls[k * 64 + 16 * bi + (lane & 15)], Rs[k * 64 + 16 * bj + (lane & 15)],
+                                                   acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int r = row0 + 4 * v;
+        if (r >= j1 && r < n && col >= j1 && col < n) Ag[(size_t)r * n + col] = acc[v];
+    }
+    __syncthreads();
+}
+
+#ifdef NRG_BLK_PROF
+#define BLK_PROF(i) do { __syncthreads(); if (tid == 0) { const long long now_ = wall_clock64(); pacc[i] += now_ - plast; plast = now_; } } while (0)
+#else
+#define BLK_PROF(i) do {} while (0)
+#endif
+// a - sum_{k < kc} (V[c][k] gam_k + p[c][k] bet_k) (coefficient k in lane k),
+// in k order; the loads
+// of 16 panel columns are issued together (clamped indices, products of
+// columns beyond kc discarded by a select, so stale values never enter)
+__device__ __forceinline__ double blk_corr(double a, const gdouble *Vb, const gdouble *Wb, int npad, int c, int kc,
+                                           double gam_l, double bet_l) {
+    for (int k0 = 0; k0 < kc; k0 += 16) {
+        double v[16], p[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int k = min(k0 + u, BLK_NB - 1);
+            v[u] = Vb[(size_t)k * npad + c];
+            p[u] = Wb[(size_t)k * npad + c];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int k = min(k0 + u, BLK_NB - 1);
+            const double t = v[u] * readlane_d(gam_l, k) + p[u] * readlane_d(bet_l, k);
+            a -= k0 + u < kc ? t : 0.0;
+        }
+    }
+    return a;
+}
+
+__device__ __noinline__ void tri_blk(const EigMat &mt, int rank, int T, int *bar, int *status, int *s_flag) {
+    extern __shared__ double lds[];
+    const int n = mt.n, npad = (n + 63) & ~63, nt = npad >> 6;
+    double *__restrict__ A = mt.A;
+    double *ws = mt.ws;
+    double *const base = (double *)(((uintptr_t)(ws + (size_t)n * n) + 511) & ~(uintptr_t)511);
+    double *const Vb = base;
+    double *const Wb = Vb + (size_t)BLK_NB * npad;
+    double *const av = Wb + (size_t)BLK_NB * npad;
+    double *const part1 = av + npad;
+    double *const part2 = part1 + (size_t)nt * BLK_P1W;
+    double *const Prow = part2 + nt;
+    double *const Pcol = Prow + (size_t)nt * nt * 64;
+    double *const tau_g = vslot(ws, n, V_TAU), *const gdg = vslot(ws, n, V_DG), *const god = vslot(ws, n, V_OD);
+    // plain accesses to global memory through global-address-space pointers
+    // (global_* instructions rather than flat_*)
+    gdouble *const gA = (gdouble *)A;
+    const gdouble *const gV = (const gdouble *)Vb, *const gW = (const gdouble *)Wb, *const gav = (const gdouble *)av;
+    const gdouble *const gPcol = (const gdouble *)Pcol;
+    const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    double *const cf = lds + BLK_LDS_CF;
+    double *const Kc = cf + 64 * CF_K;
+    double *const uW = cf + 64 * CF_UW, *const uV = cf + 64 * CF_UV;
+    double *const vj1 = cf + 64 * CF_VJ1, *const wj1 = cf + 64 * CF_WJ1, *const sc = cf + 64 * CF_SC;
+    int phase = 0, j0 = 0;
+    double tau_prev = 0.0;
+#ifdef NRG_BLK_PROF
+    long long pacc[12] = {0}, plast = wall_clock64();
+#endif
+    for (int j = 0; j < n; ++j) {
+        int kc = j - j0;
+        const int R0 = (j + 1) >> 6;  // first block of the trailing range [j+1, n)
+        // K of column j-1 = tau (p . v) / 2, p.v summed over its blocks in
+        // order: every wavefront computes it (the same bits), wavefront 0
+        // keeps it in LDS for the later phases
+        // wavefront 0 reads the partial sums and the panel's pivot-row values
+        // for the workgroup (one reader per workgroup: every rank reads the
+        // same few lines) and leaves them in LDS
+        if (w == 0) {
+            double Knew = 0.0;
+            if (kc > 0) {
+                double sp = 0.0;
+                for (int B = (j >> 6) + lane; B < nt; B += 64) sp += ld_dev(part2 + B);
+                Knew = 0.5 * tau_prev * wave_sum(sp);
+                if (lane == 0) Kc[kc - 1] = Knew;
+            }
+            const int kn = kc == BLK_NB && j < n - 1 ? 0 : kc;  // (a panel ends here: none)
+            if (lane < kn) {
+                const int k = lane;
+                const double b = ld_dev(Vb + (size_t)k * npad + j), wp = ld_dev(Wb + (size_t)k * npad + j);
+                double v1 = 0.0, p1 = 0.0;
+                if (j + 1 < n) {
+                    v1 = ld_dev(Vb + (size_t)k * npad + j + 1);
+                    p1 = ld_dev(Wb + (size_t)k * npad + j + 1);
+                }
+                const double Kk = k == kc - 1 ? Knew : Kc[k];
+                cf[64 * CF_BET + k] = b;
+                cf[64 * CF_GAM + k] = wp - 2.0 * Kk * b;
+                cf[64 * CF_WJ + k] = wp - Kk * b;
+                vj1[k] = v1;
+                wj1[k] = p1 - Kk * v1;
+            }
+        }
+        __syncthreads();
+        BLK_PROF(0);
+        // panel end: rank-2 BLK_NB update of the trailing matrix [j, n)
+        if (kc == BLK_NB && j < n - 1) {
+            const int R0s = j >> 6;
+            for (int C = R0s; C < nt; ++C) {
+                if (blk_owner(C, T) != rank) continue;
+                for (int R = R0s; R <= C; ++R) blk_syr2k_tile(A, n, npad, R, C, j, Vb, Wb, R == R0s);
+            }
+            j0 = j;
+            kc = 0;
+        }
+        BLK_PROF(1);
+        // ---- P1: the panel's earlier columns at rows j (column correction,
+        // diagonal) and j+1 (v0's share of W^T v and V^T v): lane k of every
+        // wavefront holds column k's coefficients
+        const double gam_l = lane < kc ? cf[64 * CF_GAM + lane] : 0.0, bet_l = lane < kc ? cf[64 * CF_BET + lane] : 0.0,
+                     wj_l = lane < kc ? cf[64 * CF_WJ + lane] : 0.0;
+        if (w == 0 && blk_owner(j >> 6, T) == rank) {
+            const double d = gA[(size_t)j * n + j] - 2.0 * wave_sum(bet_l * wj_l);
+            if (lane == 0) gdg[j] = d;
+        }
+        if (j == n - 1) {
+            if (rank == 0 && tid == 0) god[j] = 0.0;
+#ifdef NRG_BLK_PROF
+            if (rank == 0 && tid == 0)
+                printf("blkprof n=%d T=%d K %lld syr2k %lld p1coef %lld p1 %lld bar1 %lld p2sum %lld symv %lld bar2 %lld p3 %lld bar3 %lld\n",
+                       n, T, pacc[0], pacc[1], pacc[2], pacc[3], pacc[4], pacc[5], pacc[6], pacc[7], pacc[8], pacc[9]);
+#endif
+            break;
+        }
+        BLK_PROF(2);
+        {
+            // groups of up to 16 owned blocks: the column by one wavefront per
+            // block, then the block's partial sums (|a|^2, W^T a, V^T a) as
+            // (block, quantity) items over all wavefronts, loads issued in
+            // batches (no branch between them)
+            double *abuf = lds + BLK_LDS_XS;  // [block of the group][64]: the column's tail
+            __shared__ int gblk[16];  // the group's blocks
+            const int nq = 1 + 2 * kc;
+            for (int B0 = R0; B0 < nt;) {
+                int nb = 0, B1 = B0;
+                for (; B1 < nt && nb < 16; ++B1)
+                    if (blk_owner(B1, T) == rank) {
+                        if (tid == 0) gblk[nb] = B1;
+                        ++nb;
+                    }
+                if (nb == 0) break;
+                __syncthreads();
+                if (w < nb) {
+                    const int c = 64 * gblk[w] + lane;
+                    const bool act = c > j && c < n, tail = c > j + 1 && c < n;
+                    double a = act ? gA[(size_t)j * n + c] : 0.0;
+                    a = blk_corr(a, gV, gW, npad, c, kc, gam_l, bet_l);
+                    if (act) st_dev(av + c, a);
+                    if (tail) gA[(size_t)j * n + c] = a;
+                    abuf[w * 64 + lane] = tail ? a : 0.0;
+                }
+                __syncthreads();
+                const int ni = nb * nq;
+                for (int p0 = w; p0 < ni; p0 += 16 * 8) {
+                    double val[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int pi = min(p0 + 16 * u, ni - 1), b = pi / nq, q = pi - b * nq;
+                        const int k = max(q <= kc ? q - 1 : q - 1 - kc, 0);
+                        const int c = 64 * gblk[b] + lane;
+                        const double at = abuf[b * 64 + lane];
+                        const double v = gV[(size_t)k * npad + c], wp = gW[(size_t)k * npad + c];
+                        val[u] = q == 0 ? at * at : q <= kc ? (wp - Kc[k] * v) * at : v * at;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int pi = p0 + 16 * u;
+                        if (pi >= ni) break;
+                        const int b = pi / nq, q = pi - b * nq;
+                        const double sm = wave_sum(val[u]);
+                        const int slot = q == 0 ? 0 : q <= kc ? q : 1 + BLK_NB + (q - 1 - kc);
+                        if (lane == 0) st_dev(part1 + (size_t)gblk[b] * BLK_P1W + slot, sm);
+                    }
+                }
+                __syncthreads();
+                B0 = B1;
+            }
+        }
+        BLK_PROF(3);
+        ++phase;
+        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
+        BLK_PROF(4);
+        // ---- P2: block sums in block order (every workgroup the same bits)
+        {
+            // quantity t (0: |a|^2, then W^T a, V^T a) by wavefront t % 16,
+            // lanes over the blocks; a wavefront's loads are issued together
+            const int nq = 1 + 2 * kc;
+            constexpr int QW = (1 + 2 * BLK_NB + TRI_NW - 1) / TRI_NW;
+            double acc[QW];
+#pragma unroll
+            for (int u = 0; u < QW; ++u) acc[u] = 0.0;
+            if (tid == 0) sc[1] = ld_dev(av + j + 1);
+            for (int B = R0 + lane; B < nt; B += 64) {
+#pragma unroll
+                for (int u = 0; u < QW; ++u) {
+                    const int t = min(w + TRI_NW * u, nq - 1);
+                    const int slot = t == 0 ? 0 : t <= kc ? t : 1 + BLK_NB + (t - 1 - kc);
+                    acc[u] += ld_dev(part1 + (size_t)B * BLK_P1W + slot);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < QW; ++u) {
+                const int t = w + TRI_NW * u;
+                if (t >= nq) break;
+                const double sm = wave_sum(acc[u]);
+                if (lane == 0) {
+                    if (t == 0) sc[0] = sm;
+                    else if (t <= kc) uW[t - 1] = sm;
+                    else uV[t - 1 - kc] = sm;
+                }
+            }
+        }
+        __syncthreads();
+        const double sigma = sc[0], x1 = sc[1];
+        double tau = 0.0, v0 = 0.0, alpha = x1;
+        if (j <= n - 3 && sigma != 0.0) {
+            alpha = -copysign(sqrt(x1 * x1 + sigma), x1);
+            v0 = x1 - alpha;
+            tau = 2.0 / (v0 * v0 + sigma);
+        }
+        if (rank == 0 && tid == 0) {
+            god[j] = alpha;
+            if (j <= n - 3) tau_g[j] = tau;
+        }
+        if (tid == 0 && blk_owner((j + 1) >> 6, T) == rank) gA[(size_t)j * n + j + 1] = tau != 0.0 ? v0 : 0.0;
+        if (tid < kc) {   // W^T v and V^T v include v0 at row j+1
+            uW[tid] += wj1[tid] * v0;
+            uV[tid] += vj1[tid] * v0;
+        }
+        {
+            int idx = 0;  // column kc of the panel's V
+            for (int B = R0; B < nt; ++B) {
+                if (blk_owner(B, T) != rank) continue;
+                if ((idx++ & (TRI_NW - 1)) != w) continue;
+                const int c = 64 * B + lane;
+                double v = 0.0;
+                if (tau != 0.0 && c > j && c < n) v = c == j + 1 ? v0 : gav[c];
+                st_dev(Vb + (size_t)kc * npad + c, v);
+            }
+        }
+        BLK_PROF(5);
+        if (tau != 0.0) blk_symv(A, n, nt, R0, rank, T, j + 1, v0, av, Prow, Pcol);
+        BLK_PROF(6);
+        ++phase;
+        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
+        BLK_PROF(7);
+        // ---- P3: p on the owned blocks.  Term i of block B (row part of tile
+        // (B, B+i), then column parts of tiles (R0.., B)) is summed by
+        // wavefront i % 16; the 16 partial sums are added in wavefront order.
+        {
+            double *red = lds + BLK_LDS_TW;  // [block of the group][wavefront][64]
+            const int nterm = nt - R0;
+            for (int B0 = R0; B0 < nt;) {
+                // a group of up to 16 owned blocks [B0, B1)
+                int nb = 0, B1 = B0, mine = -1;
+                for (; B1 < nt && nb < 16; ++B1)
+                    if (blk_owner(B1, T) == rank) {
+                        if (nb == w) mine = B1;
+                        ++nb;
+                    }
+                if (nb == 0) break;
+                if (tau != 0.0) {
+                    int b = 0;
+                    for (int Bb = B0; Bb < B1; ++Bb) {
+                        if (blk_owner(Bb, T) != rank) continue;
+                        const int nrow = nt - Bb;
+                        double s = 0.0;
+                        for (int i0 = w; i0 < nterm; i0 += 16 * 16) {
+                            double t[16];
+#pragma unroll
+                            for (int u = 0; u < 16; ++u) {
+                                // both candidate loads at valid addresses, then a select
+                                // (no branch, so the 16 items' loads are in flight together)
+                                const int i = i0 + 16 * u, ic = min(i, nterm - 1);
+                                const double pr = ld_dev(Prow + ((size_t)Bb * nt + Bb + min(ic, nrow - 1)) * 64 + lane);
+                                const double pc = gPcol[((size_t)Bb * nt + R0 + max(ic - nrow, 0)) * 64 + lane];
+                                t[u] = i >= nterm ? 0.0 : ic < nrow ? pr : pc;
+                            }
+#pragma unroll
+                            for (int u = 0; u < 16; ++u) s += t[u];
+                        }
+                        red[(b * 16 + w) * 64 + lane] = s;
+                        ++b;
+                    }
+                }
+                B0 = B1;
+                __syncthreads();
+                if (mine >= 0) {
+                    const int Bb = mine;
+                    const int c = 64 * Bb + lane;
+                    const bool act = c > j && c < n;
+                    double p = 0.0, v = 0.0;
+                    if (tau != 0.0) {
+                        double y = 0.0;
+                        for (int u = 0; u < 16; ++u) y += red[(w * 16 + u) * 64 + lane];
+                        for (int k0 = 0; k0 < kc; k0 += 16) {
+                            double vv[16], pp[16];
+#pragma unroll
+                            for (int u = 0; u < 16; ++u) {
+                                const int k = min(k0 + u, BLK_NB - 1);
+                                vv[u] = gV[(size_t)k * npad + c];
+                                pp[u] = gW[(size_t)k * npad + c];
+                            }
+#pragma unroll
+                            for (int u = 0; u < 16; ++u) {
+                                const int k = min(k0 + u, BLK_NB - 1);
+                                const double t = vv[u] * uW[k] + (pp[u] - Kc[k] * vv[u]) * uV[k];
+                                y -= k0 + u < kc ? t : 0.0;
+                            }
+                        }
+                        if (act) {
+                            p = tau * y;
+                            v = c == j + 1 ? v0 : gav[c];
+                        }
+                    }
+                    st_dev(Wb + (size_t)kc * npad + c, p);
+                    const double pv = wave_sum(p * v);
+                    if (lane == 0) st_dev(part2 + Bb, pv);
+                }
+                __syncthreads();
+            }
+        }
+        BLK_PROF(8);
+        ++phase;
+        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
+        BLK_PROF(9);
+        tau_prev = tau;
+    }
+}
+
+__device__ __forceinline__ void tri_teams(const EigMat *mats, const TriJob *jobs, const int *tbase, const int *qjob,
+                                          int *counters, int *status) {
     __shared__ double red[TRI_NW];
     __shared__ int s_ticket, s_flag;
     const int x = blockIdx.x % N_XCD;
@@ -784,11 +1346,23 @@ __global__ __launch_bounds__(TRI_NT) void k_eig_tri_coop(const EigMat *mats, con
         const TriJob jb = jobs[lo];
         const EigMat mt = mats[jb.mat];
         const int rank = t0 + t - tbase[lo];
-        if (mt.n <= TEAM_LDS_N) tri_team<true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, red, &s_flag);
+        if (jb.blocked) tri_blk(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag);
+        else if (mt.n <= TEAM_LDS_N) tri_team<true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, red, &s_flag);
         else tri_team<false>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, red, &s_flag);
         __syncthreads();
     }
     }
+}
+
+// Persistent team launches (the same body): k_eig_tri_blk when some job uses
+// the blocked reduction, so that its time is reported under its own name.
+__global__ __launch_bounds__(TRI_NT) void k_eig_tri_coop(const EigMat *mats, const TriJob *jobs, const int *tbase,
+                                                         const int *qjob, int *counters, int *status) {
+    tri_teams(mats, jobs, tbase, qjob, counters, status);
+}
+__global__ __launch_bounds__(TRI_NT) void k_eig_tri_blk(const EigMat *mats, const TriJob *jobs, const int *tbase,
+                                                        const int *qjob, int *counters, int *status) {
+    tri_teams(mats, jobs, tbase, qjob, counters, status);
 }
 
 // ---------------------------------------------------------------------------
@@ -2083,7 +2657,16 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         HIPCHK(c, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
         HIPCHK(c, hipFuncSetAttribute((const void *)k_eig_tri_coop, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)TRI_COOP_LDS));
+        if (const char *e = getenv("NRG_BLK_MIN_N")) c->blk_min_n = std::max(3, atoi(e));
+        if (const char *e = getenv("NRG_BLK_TDIV")) c->blk_tdiv = std::max(1, atoi(e));
+        HIPCHK(c, hipFuncSetAttribute((const void *)k_eig_tri_blk, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)TRI_COOP_LDS));
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_coop, TRI_NT, TRI_COOP_LDS));
+        {
+            int per_cu_blk = 0;
+            HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_blk, k_eig_tri_blk, TRI_NT, TRI_COOP_LDS));
+            per_cu = std::min(per_cu, per_cu_blk);
+        }
         c->tri_capacity = std::max(1, per_cu * ncu);
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_reg, REG_NT, 0));
         // at most one team workgroup per CU: two would share its SIMDs
@@ -2099,7 +2682,8 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     std::vector<int> teamT(ntasks, 1);
     std::vector<double> cost(ntasks);
     std::vector<char> global(ntasks, 0);
-    std::vector<char> regjob(ntasks, 0);
+    std::vector<char> regjob(ntasks, 0), blkjob(ntasks, 0);
+    const int blk_min_n = c->blk_min_n, blk_tdiv = c->blk_tdiv;
     for (int i = 0; i < ntasks; ++i) {
         const double n = tasks[i].n;
         if (tasks[i].n <= REG_ONE_N) {  // one workgroup on the side stream, not queued
@@ -2112,6 +2696,22 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             teamT[i] = reg_team(tasks[i].n);
             regjob[i] = 1;
             cost[i] = 2.0 * n;
+            continue;
+        }
+        if (tasks[i].n >= blk_min_n && blk_ws_doubles(tasks[i].n) <= (size_t)tasks[i].n * tasks[i].n) {
+            // blocked reduction: about two 64-column blocks per rank; teams
+            // above one XCD's share span XCDs.  Per step three barriers, then
+            // the rank's share of the half matrix (4 (n-j)^2 bytes at ~25 GB/s
+            // per CU).
+            const int nt = (tasks[i].n + 63) / 64;
+            int T = std::max(2, nt / blk_tdiv);
+            if (T > std::min(TEAM_MAX, per_xcd)) {
+                T = std::min({T, TEAM_GLOBAL_MAX, tri_grid});
+                global[i] = 1;
+            }
+            teamT[i] = T;
+            blkjob[i] = 1;
+            cost[i] = n * (global[i] ? 12.0 : 4.5) + 5.3e-5 * n * n * n / T;
             continue;
         }
         const int nsl = (tasks[i].n - 1 + TEAM_W - 1) / TEAM_W;
@@ -2155,13 +2755,13 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         tbase.assign(1, 0);
         qjob.assign(1, 0);
         for (int i : g) {
-            jobs.push_back({i, teamT[i]});
+            jobs.push_back({i, teamT[i], blkjob[i], 0});
             tbase.push_back(tbase.back() + teamT[i]);
         }
         qjob.push_back((int)jobs.size());
         for (int x = 0; x < N_XCD; ++x) {
             for (int i : q[x]) {
-                jobs.push_back({i, teamT[i]});
+                jobs.push_back({i, teamT[i], blkjob[i], 0});
                 tbase.push_back(tbase.back() + teamT[i]);
             }
             qjob.push_back((int)jobs.size());
@@ -2214,8 +2814,11 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             double &fl = tasks[i].n <= REG_ONE_N ? fl1 : regjob[i] ? flr : flt;
             double &by = tasks[i].n <= REG_ONE_N ? by1 : regjob[i] ? byr : byt;
             fl += 4.0 / 3.0 * n * n * n;
-            by += 8.0 / 3.0 * n * n * n;
+            // the blocked product reads half the trailing matrix per column
+            by += (blkjob[i] ? 4.0 : 8.0) / 3.0 * n * n * n;
         }
+        bool any_blk = false;
+        for (int i = 0; i < ntasks; ++i) any_blk = any_blk || blkjob[i];
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(256), 0, c->stream, c->cnt, (int)ncnt);
         hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->dstatus, 1);
         HIPCHK(c, hipGetLastError());
@@ -2237,9 +2840,9 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                                c->cnt, c->dstatus, (const int *)(db1 + o_ones), (int)ones.size());
         }
         if (!jobs.empty()) {
-            LaunchTimer lt(K_EIG_TRI, c->stream, flt, byt);
-            hipLaunchKernelGGL(k_eig_tri_coop, dim3(tri_grid), dim3(TRI_NT), TRI_COOP_LDS, c->stream, d_mats, d_jobs,
-                               d_tb, d_qj, c->cnt + creg, c->dstatus);
+            LaunchTimer lt(any_blk ? K_EIG_TRI_BLK : K_EIG_TRI, c->stream, flt, byt);
+            hipLaunchKernelGGL(any_blk ? k_eig_tri_blk : k_eig_tri_coop, dim3(tri_grid), dim3(TRI_NT), TRI_COOP_LDS,
+                               c->stream, d_mats, d_jobs, d_tb, d_qj, c->cnt + creg, c->dstatus);
         }
         if (ones_side) {
             HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));

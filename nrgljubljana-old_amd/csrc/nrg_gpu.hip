@@ -17,7 +17,7 @@
 
 #include "nrg_internal.h"
 
-const char *kKernelNames[K_COUNT] = {"k_zero",       "k_assemble", "k_eig_tri_coop", "k_eig_tri_one", "k_eig_tri_reg", "k_dc_leaf", "k_dc_small", "k_dc_deflate",
+const char *kKernelNames[K_COUNT] = {"k_zero",       "k_assemble", "k_eig_tri_coop", "k_eig_tri_one", "k_eig_tri_reg", "k_eig_tri_blk", "k_dc_leaf", "k_dc_small", "k_dc_deflate",
                                      "k_dc_secular", "k_dc_zhat",  "k_dc_unorm", "k_gemm_dc", "k_gemm_bt", "k_dc_rank",
                                      "k_eig_bt",     "k_eig_copy", "k_gemm",     "k_spec_bin", "k_gather_scatter"};
 bool g_timing = false;

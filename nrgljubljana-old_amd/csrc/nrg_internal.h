@@ -41,6 +41,10 @@ struct nrg_ctx {
     size_t cnt_ints = 0;
     int tri_capacity = 0;
     int reg_capacity = 0;
+    // matrices from this dimension up use the blocked tridiagonalisation
+    int blk_min_n = 1024;
+    // blocked teams: one workgroup per blk_tdiv 64-column blocks
+    int blk_tdiv = 4;
     // grouped GEMM: partial tiles of split outputs
     double *gpart = nullptr;
     size_t gpart_tiles = 0;
@@ -82,6 +86,7 @@ enum KernelId {
     K_EIG_TRI,
     K_EIG_TRI1,
     K_EIG_TRI_REG,
+    K_EIG_TRI_BLK,
     K_DC_LEAF,
     K_DC_SMALL,
     K_DC_DEFLATE,
