@@ -2871,6 +2871,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     struct Level {
         int m0, m1, s0, s1, u0, u1, g0, g1, x0, x1, c0, c1, r0, r1, maxK;
         double flops;
+        bool mfma;
     };
     std::vector<Level> levels(maxlev + 1);
     // levels 1..DC_SMALL_LEV: one k_dc_small workgroup per subtree
@@ -2887,6 +2888,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         lv.g0 = (int)gitems.size();
         lv.x0 = (int)gathers.size();
         lv.c0 = (int)copies.size();
+        const size_t lo0 = gouts.size();
         for (int i = 0; i < nbig; ++i) {
             const int n = tasks[i].n, nlev = dc_levels(n);
             if (L > nlev) continue;
@@ -2922,8 +2924,6 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                 for (int s0 = 0; s0 < nseg_max; s0 += 4)
                     for (int c0 = 0; c0 < K; c0 += 64) gathers.push_back({mi, s0, c0, 0});
                 for (int j0 = 0; j0 < K; j0 += COPY_ROWS) copies.push_back({mi, j0});
-                for (int tm = 0; tm * GT < K; ++tm)
-                    for (int tn = 0; tn * GT < K; ++tn) gitems.push_back({g, tm, tn, 0, 0, 0});
                 lv.maxK = std::max(lv.maxK, K);
                 lv.flops += 2.0 * K * K * K;
             }
@@ -2932,7 +2932,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         lv.s1 = (int)schunks.size();
         lv.u1 = (int)uchunks.size();
         lv.r0 = (int)gred.size();
-        gslots = std::max(gslots, gemm_split(gitems, lv.g0, gks, gred));
+        lv.mfma = gemm_plan(gouts, lo0, gks, gitems, gred, gslots);
         lv.r1 = (int)gred.size();
         lv.g1 = (int)gitems.size();
         lv.x1 = (int)gathers.size();
@@ -2970,6 +2970,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         int w0, w1;
         int g[4][2];  // gemm item ranges of G, M, W, Z
         int r[4][2];  // reduce item ranges
+        bool mfma[4];
         double flops;
     };
     std::vector<WyLevel> wylev(wy_blocks);
@@ -2987,6 +2988,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         L.w1 = (int)wyitems.size();
         for (int stage = 0; stage < 4; ++stage) {
             L.g[stage][0] = (int)wygi.size();
+            const size_t so0 = wyouts.size();
             for (int w = L.w0; w < L.w1; ++w) {
                 const WyItem &it = wyitems[w];
                 const int n = it.n, j0 = it.j0, nb = it.nb, np = n - j0 - 1;
@@ -3015,12 +3017,11 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                 wyouts.push_back(o);
                 wyterms.push_back(t);
                 wyks.push_back((t.k + GK - 1) / GK);
-                for (int tm = 0; tm * GT < o.m; ++tm)
-                    for (int tn = 0; tn * GT < o.n; ++tn) wygi.push_back({g, tm, tn, 0, 0, 0});
+                (void)g;
                 if (stage >= 2) L.flops += 2.0 * n * (double)nb * np;
             }
             L.r[stage][0] = (int)wyred.size();
-            gslots = std::max(gslots, gemm_split(wygi, L.g[stage][0], wyks, wyred));
+            L.mfma[stage] = gemm_plan(wyouts, so0, wyks, wygi, wyred, gslots);
             L.r[stage][1] = (int)wyred.size();
             L.g[stage][1] = (int)wygi.size();
         }
@@ -3117,7 +3118,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         }
         HIPCHK(c, hipGetLastError());
         s = gemm_launch_dev(c, K_GEMM_DC, d_gouts, d_gterms, d_gitems + lv.g0, lv.g1 - lv.g0, d_gred + lv.r0,
-                            lv.r1 - lv.r0, lv.flops, 0.0);
+                            lv.r1 - lv.r0, lv.flops, 0.0, lv.mfma);
         if (s) return s;
     }
     // 3. sort, back-transformation, copy into A
@@ -3176,7 +3177,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         auto wy_gemm = [&](const WyLevel &L, int stage, double fl) {
             return gemm_launch_dev(c, K_GEMM_BT, d_wyouts, d_wyterms, d_wygi + L.g[stage][0],
                                    L.g[stage][1] - L.g[stage][0], d_wyred + L.r[stage][0],
-                                   L.r[stage][1] - L.r[stage][0], fl, 0.0);
+                                   L.r[stage][1] - L.r[stage][0], fl, 0.0, L.mfma[stage]);
         };
         {
             LaunchTimer lt(K_EIG_BT, c->stream, 0.0, 0.0);

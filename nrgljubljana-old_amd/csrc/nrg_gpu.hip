@@ -233,16 +233,18 @@ __global__ __launch_bounds__(256) void k_assemble(const nrg_asm_block *blocks, c
 constexpr int GLQ = GT * GK / 256;  // tile elements per thread and operand
 constexpr int GMT = GT / 16;        // outputs per thread along each tile dimension
 
-// op(A) rows row0.., k k0.. and op(B) k k0.., columns col0.. of one step
+// op(A) rows row0.., k k0.. and op(B) k k0.., columns col0.. of one step of a
+// TILE x TILE output tile (TILE * GK / 256 elements per thread and operand)
+template <int TILE, int NQ = TILE * GK / 256>
 __device__ __forceinline__ void gemm_load_step(const nrg_gemm_term &tm, int m, int n, int row0, int col0, int k0,
-                                               int tid, double (&ra)[GLQ], double (&rb)[GLQ]) {
+                                               int tid, double (&ra)[NQ], double (&rb)[NQ]) {
 #pragma unroll
-    for (int q = 0; q < GLQ; ++q) {
+    for (int q = 0; q < NQ; ++q) {
         const int e = tid + q * 256;
         int r, kk;
         if (tm.transA) {  // contiguous along rows of op(A)
-            kk = e / GT;
-            r = e % GT;
+            kk = e / TILE;
+            r = e % TILE;
         } else {
             r = e / GK;
             kk = e % GK;
@@ -256,15 +258,15 @@ __device__ __forceinline__ void gemm_load_step(const nrg_gemm_term &tm, int m, i
         ra[q] = v;
     }
 #pragma unroll
-    for (int q = 0; q < GLQ; ++q) {
+    for (int q = 0; q < NQ; ++q) {
         const int e = tid + q * 256;
         int cidx, kk;
         if (tm.transB) {
             cidx = e / GK;
             kk = e % GK;
         } else {
-            kk = e / GT;
-            cidx = e % GT;
+            kk = e / TILE;
+            cidx = e % TILE;
         }
         const int gc = col0 + cidx, gk = k0 + kk;
         double v = 0.0;
@@ -273,15 +275,16 @@ __device__ __forceinline__ void gemm_load_step(const nrg_gemm_term &tm, int m, i
     }
 }
 
-__device__ __forceinline__ void gemm_store_step(const nrg_gemm_term &tm, int tid, const double (&ra)[GLQ],
-                                                const double (&rb)[GLQ], double (*As)[GT + 1], double (*Bs)[GT + 1]) {
+template <int TILE, int NQ = TILE * GK / 256>
+__device__ __forceinline__ void gemm_store_step(const nrg_gemm_term &tm, int tid, const double (&ra)[NQ],
+                                                const double (&rb)[NQ], double (*As)[TILE + 1], double (*Bs)[TILE + 1]) {
 #pragma unroll
-    for (int q = 0; q < GLQ; ++q) {
+    for (int q = 0; q < NQ; ++q) {
         const int e = tid + q * 256;
-        if (tm.transA) As[e / GT][e % GT] = ra[q];
+        if (tm.transA) As[e / TILE][e % TILE] = ra[q];
         else As[e % GK][e / GK] = ra[q];
         if (tm.transB) Bs[e % GK][e / GK] = rb[q];
-        else Bs[e / GT][e % GT] = rb[q];
+        else Bs[e / TILE][e % TILE] = rb[q];
     }
 }
 
@@ -325,10 +328,10 @@ __global__ __launch_bounds__(256) void k_gemm(const nrg_gemm_out *outs, const nr
         break;
     }
     double ra[GLQ], rb[GLQ];
-    if (have) gemm_load_step(tm, o.m, o.n, row0, col0, k0, tid, ra, rb);
+    if (have) gemm_load_step<GT>(tm, o.m, o.n, row0, col0, k0, tid, ra, rb);
     int buf = 0;
     while (have) {
-        gemm_store_step(tm, tid, ra, rb, As[buf], Bs[buf]);
+        gemm_store_step<GT>(tm, tid, ra, rb, As[buf], Bs[buf]);
         __syncthreads();
         // the next step (possibly of a later term), loaded while this one is computed
         ++g;
@@ -345,7 +348,7 @@ __global__ __launch_bounds__(256) void k_gemm(const nrg_gemm_out *outs, const nr
                 }
             }
         }
-        if (next) gemm_load_step(tm, o.m, o.n, row0, col0, k0, tid, ra, rb);
+        if (next) gemm_load_step<GT>(tm, o.m, o.n, row0, col0, k0, tid, ra, rb);
 #pragma unroll
         for (int kk = 0; kk < GK; ++kk) {
             double a[GMT], b[GMT];
@@ -400,6 +403,109 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(const nrg_gemm_out *outs, c
         double *cp = o.C + (size_t)gr * o.ldc + gc;
         *cp = (o.beta == 0.0 ? 0.0 : o.beta * *cp) + v;
     }
+}
+
+// ---------------------------------------------------------------------------
+// grouped GEMM on FP64 MFMA for launches with many output tiles: one 64 x 64
+// tile per workgroup of four wavefronts, each wavefront a 32 x 32 quarter as
+// 2 x 2 blocks of v_mfma_f64_16x16x4f64 (A[i][k] in lane i + 16k, B[k][j] in
+// lane j + 16k, D[i][j] in lane j + 16 (i % 4), element i / 4).  The same
+// per-output term lists and operand staging (two LDS buffers, one barrier per
+// k-step of GK) as k_gemm; tiles are never split, so the accumulation order
+// of an output is fixed.
+// ---------------------------------------------------------------------------
+typedef double gemm_d4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_gemm_mfma(const nrg_gemm_out *outs, const nrg_gemm_term *terms,
+                                                   const GemmItem *items) {
+    __shared__ double As[2][GK][GTM + 1];
+    __shared__ double Bs[2][GK][GTM + 1];
+    const GemmItem it = items[blockIdx.x];
+    const nrg_gemm_out o = outs[it.out];
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int wr = w >> 1, wc = w & 1;
+    const int row0 = it.tm * GTM, col0 = it.tn * GTM;
+    if (row0 >= o.m || col0 >= o.n) return;  // m may have been lowered on the device
+    gemm_d4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = gemm_d4{0.0, 0.0, 0.0, 0.0};
+    int t = 0, k0 = 0;
+    nrg_gemm_term tm{};
+    bool have = false;
+    for (; t < o.term_count; ++t) {
+        tm = terms[o.term_begin + t];
+        if (tm.k > 0) {
+            have = true;
+            break;
+        }
+    }
+    constexpr int NQ = GTM * GK / 256;
+    double ra[NQ], rb[NQ];
+    if (have) gemm_load_step<GTM>(tm, o.m, o.n, row0, col0, 0, tid, ra, rb);
+    int buf = 0;
+    const int li = lane & 15, lk = lane >> 4;
+    while (have) {
+        gemm_store_step<GTM>(tm, tid, ra, rb, As[buf], Bs[buf]);
+        __syncthreads();
+        k0 += GK;
+        bool next = true;
+        if (k0 >= tm.k) {
+            next = false;
+            for (++t; t < o.term_count; ++t) {
+                tm = terms[o.term_begin + t];
+                if (tm.k > 0) {
+                    next = true;
+                    k0 = 0;
+                    break;
+                }
+            }
+        }
+        if (next) gemm_load_step<GTM>(tm, o.m, o.n, row0, col0, k0, tid, ra, rb);
+#pragma unroll
+        for (int kk = 0; kk < GK; kk += 4) {
+            const double a0 = As[buf][kk + lk][wr * 32 + li], a1 = As[buf][kk + lk][wr * 32 + 16 + li];
+            const double b0 = Bs[buf][kk + lk][wc * 32 + li], b1 = Bs[buf][kk + lk][wc * 32 + 16 + li];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        buf ^= 1;
+        have = next;
+    }
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gr = row0 + wr * 32 + bi * 16 + lk + 4 * v, gc = col0 + wc * 32 + bj * 16 + li;
+                if (gr >= o.m || gc >= o.n) continue;
+                double *cp = o.C + (size_t)gr * o.ldc + gc;
+                *cp = (o.beta == 0.0 ? 0.0 : o.beta * *cp) + acc[bi][bj][v];
+            }
+}
+
+bool gemm_plan(const std::vector<nrg_gemm_out> &outs, size_t o0, const std::vector<int> &ksteps,
+               std::vector<GemmItem> &items, std::vector<GemmItem> &reduce, int &slots) {
+    size_t big = 0;
+    for (size_t o = o0; o < outs.size(); ++o)
+        big += (size_t)((outs[o].m + GTM - 1) / GTM) * ((outs[o].n + GTM - 1) / GTM);
+    if (big >= (size_t)GEMM_MFMA_MIN_TILES) {
+        for (size_t o = o0; o < outs.size(); ++o)
+            for (int tm = 0; tm * GTM < outs[o].m; ++tm)
+                for (int tn = 0; tn * GTM < outs[o].n; ++tn) items.push_back({(int)o, tm, tn, 0, 0, 0});
+        return true;
+    }
+    const size_t first = items.size();
+    for (size_t o = o0; o < outs.size(); ++o)
+        for (int tm = 0; tm * GT < outs[o].m; ++tm)
+            for (int tn = 0; tn * GT < outs[o].n; ++tn) items.push_back({(int)o, tm, tn, 0, 0, 0});
+    slots = std::max(slots, gemm_split(items, first, ksteps, reduce));
+    return false;
 }
 
 std::vector<int> gemm_ksteps(const nrg_gemm_out *outs, int nouts, const nrg_gemm_term *terms) {
@@ -510,9 +616,14 @@ __global__ __launch_bounds__(256) void k_spec_bin(const nrg_spec_params P, const
 
 nrg_status gemm_launch_dev(nrg_ctx *c, int kid, const nrg_gemm_out *d_outs, const nrg_gemm_term *d_terms,
                            const GemmItem *d_items, int nitems, const GemmItem *d_reduce, int nreduce,
-                           double flops, double bytes) {
+                           double flops, double bytes, bool mfma) {
     if (nitems <= 0) return NRG_OK;
     LaunchTimer lt(kid, c->stream, flops, bytes);
+    if (mfma) {
+        hipLaunchKernelGGL(k_gemm_mfma, dim3(nitems), dim3(256), 0, c->stream, d_outs, d_terms, d_items);
+        HIPCHK(c, hipGetLastError());
+        return NRG_OK;
+    }
     hipLaunchKernelGGL(k_gemm, dim3(nitems), dim3(256), 0, c->stream, d_outs, d_terms, d_items, c->gpart);
     if (nreduce > 0)
         hipLaunchKernelGGL(k_gemm_reduce, dim3(nreduce), dim3(256), 0, c->stream, d_outs, d_reduce, c->gpart);
@@ -790,12 +901,12 @@ nrg_status nrg_gemm_grouped(nrg_ctx *c, const nrg_gemm_out *outs, int nouts, con
             const int needA = T.transA ? O.m : T.k, needB = T.transB ? T.k : O.n;
             if (T.k < 0 || T.lda < needA || T.ldb < needB) return fail(c, NRG_ERR_ARG, "gemm: bad term");
         }
-        for (int tm = 0; tm * GT < O.m; ++tm)
-            for (int tn = 0; tn * GT < O.n; ++tn) items.push_back({o, tm, tn, 0, 0, 0});
     }
-    if (items.empty()) return NRG_OK;
     std::vector<GemmItem> reduce;
-    const int slots = gemm_split(items, 0, gemm_ksteps(outs, nouts, terms), reduce);
+    int slots = 0;
+    const bool mfma = gemm_plan(std::vector<nrg_gemm_out>(outs, outs + nouts), 0, gemm_ksteps(outs, nouts, terms),
+                                items, reduce, slots);
+    if (items.empty()) return NRG_OK;
     if (slots) {
         nrg_status s = gemm_reserve(c, slots);
         if (s) return s;
@@ -820,7 +931,7 @@ nrg_status nrg_gemm_grouped(nrg_ctx *c, const nrg_gemm_out *outs, int nouts, con
     for (int o = 0; o < nouts; ++o) gbytes += 8.0 * outs[o].m * outs[o].n * (outs[o].beta != 0.0 ? 2 : 1);
     return gemm_launch_dev(c, K_GEMM, (const nrg_gemm_out *)d, (const nrg_gemm_term *)((char *)d + ob),
                            (const GemmItem *)((char *)d + ob + tb), (int)items.size(),
-                           (const GemmItem *)((char *)d + ob + tb + ib), (int)reduce.size(), gflops, gbytes);
+                           (const GemmItem *)((char *)d + ob + tb + ib), (int)reduce.size(), gflops, gbytes, mfma);
 }
 
 nrg_status nrg_spec_bin(nrg_ctx *c, const nrg_spec_params *prm, const nrg_spec_pair *pairs, int npairs,

@@ -167,12 +167,10 @@ def test_diag_batch_many_small_with_shared_spectrum(ctx):
         _check_eig(A, W, U)
 
 
-def test_gemm_grouped_all_transposes_and_scales(ctx):
+def _gemm_cases(ctx, rng, shapes):
     capi = _capi()
-    rng = np.random.default_rng(4)
     outs, terms, refs, handles = [], [], [], []
-    for case in range(12):
-        m, n = rng.integers(1, 150, 2)
+    for case, (m, n) in enumerate(shapes):
         ldc = n + (case % 3)
         Cinit = rng.standard_normal((m, ldc))
         beta = [0.0, 1.0, -0.5][case % 3]
@@ -183,7 +181,7 @@ def test_gemm_grouped_all_transposes_and_scales(ctx):
         o.term_begin = len(terms)
         nt = 1 + case % 3
         for t_ in range(nt):
-            k = int(rng.integers(1, 90))
+            k = int(rng.integers(1, 90 if m * n < 40000 else 700))
             tA, tB = bool(rng.integers(0, 2)), bool(rng.integers(0, 2))
             lda_extra, ldb_extra = int(rng.integers(0, 3)), int(rng.integers(0, 3))
             A = rng.standard_normal((k, m + lda_extra) if tA else (m, k + lda_extra))
@@ -209,6 +207,21 @@ def test_gemm_grouped_all_transposes_and_scales(ctx):
         got = ctx.get(p, (m, ldc))
         np.testing.assert_allclose(got[:, :n], ref, rtol=1e-13, atol=1e-12)
         np.testing.assert_array_equal(got[:, n:], Cinit[:, n:])  # outside the view untouched
+
+
+def test_gemm_grouped_all_transposes_and_scales(ctx):
+    # few tiles: the 32 x 32 FMA kernel
+    rng = np.random.default_rng(4)
+    _gemm_cases(ctx, rng, [tuple(rng.integers(1, 150, 2)) for _ in range(12)])
+
+
+def test_gemm_grouped_mfma_tiles(ctx):
+    # at least 256 tiles of 64 x 64 in the launch: the FP64-MFMA kernel, with
+    # ragged edges (sizes not multiples of 64 or 16), every transpose, scale
+    # vectors, several terms and beta
+    rng = np.random.default_rng(14)
+    shapes = [(1100, 1030), (577, 700), (64, 4096), (130, 65), (1, 900), (333, 1), (1023, 1025)]
+    _gemm_cases(ctx, rng, shapes)
 
 
 def test_gemm_grouped_split_long_k(ctx):
