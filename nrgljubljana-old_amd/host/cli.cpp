@@ -26,7 +26,7 @@ static std::string exe_dir() {
     return std::string(dirname(buf));
 }
 
-extern "C" int nrg_main(int argc, char **argv) {
+static int nrg_main_args(int argc, char **argv) {
     nrg::Options opt;
     opt.tables_dir = exe_dir() + "/../tables";
     if (const char *t = std::getenv("NRG_TABLES")) opt.tables_dir = t;
@@ -46,10 +46,25 @@ extern "C" int nrg_main(int argc, char **argv) {
         else if (a == "--plain-diag") opt.plain = true;
         else if (a == "--hbm-budget") opt.hbm_budget_gb = std::atof(next().c_str());
         else if (a == "--spill-all") opt.hbm_budget_gb = 0.0;
+        else if (a == "--iter-log") opt.iter_log = next();
         else if (a == "-h" || a == "--help") {
-            std::cout << "usage: nrg [-w workdir] [--dump-eigen file] [--stats file [--stats-light]] [--tables dir] [--device n] [--quiet] [--plain-diag] [--hbm-budget GB] [--spill-all]\n";
+            std::cout << "usage: nrg [-w workdir] [--dump-eigen file] [--stats file [--stats-light]] [--tables dir] [--device n] [--quiet] [--plain-diag] [--hbm-budget GB] [--spill-all] [--iter-log file]\n";
             return 0;
         } else nrg::die("unknown option " + a);
     }
     return nrg::run(opt);
+}
+
+// Status 0, or 1 after an error or a requested stop, whose message goes to
+// stderr (the process and the GPU context stay usable for in-process callers).
+extern "C" int nrg_main(int argc, char **argv) {
+    try {
+        return nrg_main_args(argc, argv);
+    } catch (const nrg::Stop &e) {
+        std::cout << e.what() << std::endl;
+        return 1;
+    } catch (const std::exception &e) {
+        std::cerr << "Error: " << e.what() << std::endl;
+        return 1;
+    }
 }

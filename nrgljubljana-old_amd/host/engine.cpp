@@ -287,6 +287,13 @@ struct Engine {
     size_t bin_capacity = 0;
 
     explicit Engine(const Options &o) : opt(o), lease(o.device), gpu(*lease.g) {}
+    // after an error: queued work drains before the memory goes back to the
+    // pool, and spilled eigenvectors release their pinned host memory
+    ~Engine() {
+        (void)nrg_sync(gpu.ctx);
+        for (auto &it : iters)
+            if (it.host) (void)nrg_host_free(gpu.ctx, it.host);
+    }
 
     double SCALE(int N) const {
         double s = 0.5 * (1.0 + 1.0 / P.Lambda);
@@ -858,6 +865,149 @@ struct Engine {
         }
     }
 
+    // energies.nrg / energies.dmnrg (dumptofile, src/nrg.cc:1435-1443): all
+    // (pre-truncation) energies of an iteration, one line per subspace
+    static void dump_energies(std::ofstream &f, int N, const std::map<Invar, std::vector<double>> &vals) {
+        if (!f.is_open()) return;
+        f << "\n===== Iteration number: " << N << "\n";
+        for (auto &kv : vals) {
+            f << "Subspace: " << kv.first.first << " " << kv.first.second << "\n";
+            for (double x : kv.second) f << x << ' ';
+            f << "\n";
+        }
+    }
+
+    // my_fcmp (src/numerics.h:37-53): equal if both are zero, both below
+    // eps, or relatively closer than eps
+    static int fcmp(double x, double y, double eps) {
+        if (x == 0.0 && y == 0.0) return 0;
+        if (std::fabs(x) < eps && std::fabs(y) < eps) return 0;
+        if (std::fabs(x - y) / (std::fabs(x) + std::fabs(y)) < eps) return 0;
+        return x > y ? 1 : -1;
+    }
+
+    // annotated.dat (dump_annotated, src/nrg.cc:2062-2107): the lowest
+    // dumpannotated levels of an iteration (extended over a cluster at the
+    // cut), grouped by degeneracy with their quantum numbers and total
+    // multiplicity; iterations separated by an empty line
+    std::ofstream fannotated;
+    void dump_annotated(const std::map<Invar, std::vector<double>> &vals, double scale, double totalE) {
+        if (!fannotated.is_open()) return;
+        std::vector<std::pair<double, Invar>> lst;
+        for (auto &kv : vals)
+            for (double e : kv.second) lst.push_back({e, kv.first});
+        std::sort(lst.begin(), lst.end());
+        size_t len = std::min(lst.size(), (size_t)P.dumpannotated);
+        for (size_t i = len; i + 1 < lst.size(); ++i) {  // the reference stops before the last level
+            if (fcmp(lst[i].first, lst[i - 1].first, P.grouptol) == 0) ++len;
+            else break;
+        }
+        auto en = [&](double e) { return e * (P.dumpscaled ? 1.0 : scale) + (P.dumpabs ? totalE : 0.0); };
+        auto qn = [](const Invar &I) { return std::to_string(I.first) + " " + std::to_string(I.second); };
+        fannotated << std::setprecision(P.dumpprecision);
+        if (P.dumpgroups) {
+            for (size_t i = 0; i < len;) {
+                fannotated << en(lst[i].first);
+                std::vector<std::string> q;
+                long deg = 0;
+                const size_t i0 = i;
+                while (i < len && fcmp(lst[i].first, lst[i0].first, P.grouptol) == 0) {
+                    q.push_back(qn(lst[i].second));
+                    deg += T.mult(lst[i].second);
+                    ++i;
+                }
+                std::sort(q.begin(), q.end());
+                for (auto &x : q) fannotated << " (" << x << ")";
+                fannotated << " [" << deg << "]\n";
+            }
+        } else {
+            for (auto &x : lst) fannotated << en(x.first) << " " << qn(x.second) << "\n";
+        }
+        fannotated << "\n";
+    }
+
+    // subspaces.dat (dump_subspace_information, src/nrg.cc:3207-3218)
+    void dump_subspaces() {
+        std::ofstream o(path("subspaces.dat"));
+        for (int N = P.Ninit; N < Nmax; ++N) {
+            o << "Iteration " << N << "\n";
+            o << "len_dm=" << iters[N].subs.size() << "\n";
+            for (auto &kv : iters[N].subs)
+                o << "I=" << kv.first.first << " " << kv.first.second << " len=" << kv.second.kept
+                  << " kept=" << kv.second.kept << " total=" << kv.second.d << "\n";
+            o << "\n";
+        }
+    }
+
+    // customfdm (nrg_measure_singlet1_fdm, calc_trace_fdm_kept,
+    // src/nrg.cc:973-991, 2239-2247, 3241-3245): <O> = sum_I mult(I)
+    // sum_{i,j<kept} rho_FDM[I](i,j) O[I](j,i) at iteration fdmexpvn, with
+    // the singlet operators' diagonal blocks kept from the first pass
+    std::map<std::string, std::map<Invar, std::vector<double>>> fdmexpv_ops;  // kept x kept blocks
+    void keep_fdmexpv_ops(const Iter &it, const std::map<std::string, DevOp> &ops) {
+        for (auto &name : singlet_names) {
+            const DevOp &op = ops.at(name);
+            for (auto &kv : it.subs) {
+                const int k = kv.second.kept;
+                std::vector<double> m((size_t)k * k);
+                if (k) {
+                    auto b = op.find({kv.first, kv.first});
+                    if (b == op.end()) die("fdmexpv: missing block");
+                    std::vector<nrg_strided> rows;
+                    for (int r = 0; r < k; ++r) rows.push_back({b->second.p + (size_t)r * b->second.ld, k, 1});
+                    gpu.gather(rows, m.data());
+                }
+                fdmexpv_ops[name][kv.first] = std::move(m);
+            }
+        }
+    }
+    void write_customfdm() {
+        const int N = P.fdmexpvn;
+        if (N < P.Ninit || N >= Nmax - 1) die("fdmexpvn must lie in [Ninit, Nmax-1)");
+        std::map<Invar, std::vector<double>> rho;
+        for (auto &kv : rhofdm[N]) {
+            const int k = kv.second.dim;
+            std::vector<double> m((size_t)k * k, 0.0);
+            if (k) {
+                if (kv.second.diag) {
+                    std::vector<double> d(k);
+                    gpu.gather({{kv.second.p, k, 1}}, d.data());
+                    for (int i = 0; i < k; ++i) m[(size_t)i * k + i] = d[i];
+                } else {
+                    gpu.gather({{kv.second.p, (int64_t)k * k, 1}}, m.data());
+                }
+            }
+            rho[kv.first] = std::move(m);
+        }
+        std::ofstream f(path("customfdm"));
+        auto fo = [&](auto x) { f << std::setw(P.width_custom) << std::setprecision(P.prec_custom) << x << ' '; };
+        f << '#';
+        fo(1);
+        for (size_t k = 1; k <= singlet_names.size(); ++k) fo(1 + k);
+        f << '\n' << '#';
+        fo("T");
+        for (auto &n : singlet_names) fo(n);
+        f << '\n' << ' ';
+        fo(P.T);
+        for (auto &name : singlet_names) {
+            double tr = 0.0;
+            for (auto &kv : iters[N].subs) {
+                const int k = kv.second.kept;
+                const std::vector<double> &r = rho.at(kv.first), &o = fdmexpv_ops.at(name).at(kv.first);
+                double sum = 0.0;
+                for (int i = 0; i < k; ++i)
+                    for (int j = 0; j < k; ++j) sum += r[(size_t)i * k + j] * o[(size_t)j * k + i];
+                tr += T.mult(kv.first) * sum;
+            }
+            fo(tr);
+        }
+        f << '\n';
+    }
+
+    // first-pass iteration log for timing (stream drained at each line)
+    std::ofstream fiterlog;
+    std::chrono::steady_clock::time_point t_start;
+
     std::function<void()> deferred;  // pass 1: host-side output of the previous iteration
     void pass1() {
         State st;
@@ -865,16 +1015,30 @@ struct Engine {
         open_td();
         open_custom();
         if (P.dumpenergies) fenergies.open(path("energies.nrg"));
+        if (P.dumpannotated > 0) fannotated.open(path("annotated.dat"));
+        if (!opt.iter_log.empty()) fiterlog.open(opt.iter_log);
         double totalenergy = D.GS;
         iters.clear();
         iters.resize(Nmax);
         if (P.calc0) {
+            // docalc0 (src/nrg.cc:3064-3075): measurements on the initial
+            // cluster, labelled iteration Ninit - 1
             std::map<Invar, std::vector<double>> v0(D.energies.begin(), D.energies.end());
             td_row(v0, SCALE(0));
+            dump_annotated(v0, SCALE(0), totalenergy);
             custom_row(SCALE(0) / P.betabar, v0, st.ops_s);
+            dump_energies(fenergies, P.Ninit - 1, v0);
         }
         for (int N = P.Ninit; N < Nmax; ++N) {
             const bool last = (N + 1 == Nmax);
+            if (P.forcestop == N) {  // src/nrg.cc:3113-3114
+                if (deferred) {
+                    deferred();
+                    deferred = nullptr;
+                }
+                flush_custom();
+                throw Stop("*** Stop request at iteration " + std::to_string(N));
+            }
             Iter &it = iters[N];
             it.N = N;
             it.scale = SCALE(N + 1);
@@ -1070,16 +1234,10 @@ struct Engine {
                     allv[kv.first] = s.value;
                     keptv[kv.first] = std::vector<double>(s.value.begin(), s.value.begin() + s.kept);
                 }
-                if (fenergies.is_open()) {
-                    fenergies << "\n===== Iteration number: " << N << "\n";
-                    for (auto &kv : allv) {
-                        fenergies << "Subspace: " << kv.first.first << " " << kv.first.second << "\n";
-                        for (double x : kv.second) fenergies << x << ' ';
-                        fenergies << "\n";
-                    }
-                }
+                dump_energies(fenergies, N, allv);
                 write_dump_iter(it);
                 td_row(allv, it.scale);
+                dump_annotated(allv, it.scale, te);
                 size_t nkept = 0, nall = 0;
                 for (auto &kv : it.subs) {
                     nkept += kv.second.kept;
@@ -1099,6 +1257,15 @@ struct Engine {
             for (auto &kv : it.subs) np[kv.first] = Prev{kv.second.kept, kv.second.W};
             st.prev = std::move(np);
             if (N == P.Ninit) plan_schedule();
+            if (P.fdmexpv && N == P.fdmexpvn) keep_fdmexpv_ops(it, st.ops_s);
+            if (fiterlog.is_open()) {
+                gpu.check(nrg_sync(gpu.ctx), "nrg_sync");
+                long long ep = 0;
+                for (auto &kv : it.subs) ep += kv.second.d;
+                fiterlog << N << ' ' << std::setprecision(9)
+                         << std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count() << ' '
+                         << ep << std::endl;
+            }
         }
         pass1_N = -1;
         GSenergy = totalenergy;
@@ -1107,6 +1274,8 @@ struct Engine {
         ftd.close();
         flush_custom();
         fcustom.close();
+        fannotated.close();
+        if (P.dumpsubspaces) dump_subspaces();  // finalize_nrg (src/nrg.cc:3222-3234)
     }
 
     // ------------------------------------------------------------ DM sweeps
@@ -1494,10 +1663,20 @@ struct Engine {
         }
         double totalenergy = D.GS;
         const bool recalc_all = P.fdm;  // do_recalc_all (src/nrg.cc:2436-2444)
+        // energies.dmnrg: the second run's dump, to be compared with
+        // energies.nrg (src/nrg.cc:1805-1810)
+        std::ofstream fen2;
+        if (P.dumpenergies) fen2.open(path("energies.dmnrg"));
+        if (P.calc0) dump_energies(fen2, P.Ninit - 1, std::map<Invar, std::vector<double>>(D.energies.begin(), D.energies.end()));
         for (int N = P.Ninit; N < Nmax; ++N) {
             const bool last = (N + 1 == Nmax);
             use(N);
             Iter &it = iters[N];
+            if (fen2.is_open()) {
+                std::map<Invar, std::vector<double>> allv;
+                for (auto &kv : it.subs) allv[kv.first] = kv.second.value;
+                dump_energies(fen2, N, allv);
+            }
             // the stored values are already shifted by Egs and fixed
             totalenergy += it.Egs * it.scale;
             truncate_prepare(it, last, true);
@@ -1672,15 +1851,28 @@ struct Engine {
             nrg_timing_enable(1);
             nrg_timing_reset();
         }
+        t_start = t0;
         setup();
         if (!opt.dump_eigen.empty()) fdump.open(opt.dump_eigen, std::ios::binary);
-        if (opt.diag) opt.diag->enter();
-        pass1();
-        if (opt.diag) opt.diag->leave();
+        {
+            // the shared eigensolver expects this solve until it leaves,
+            // also when the pass ends with an error
+            struct Leave {
+                DiagService *d;
+                ~Leave() {
+                    if (d) d->leave();
+                }
+            } leave{opt.diag};
+            if (opt.diag) opt.diag->enter();
+            pass1();
+        }
+        if (P.stopafter == "nrg") throw Stop("*** Stopped after the first sweep.");  // src/nrg.cc:3453
         if (P.dm) {
             dm_sweeps();
+            if (P.stopafter == "rho") throw Stop("*** Stopped after the DM calculation.");  // src/nrg.cc:3462
             pass2();
             write_spectra();
+            if (P.fdmexpv) write_customfdm();
         }
         for (auto &it : iters)
             if (it.host) gpu.check(nrg_host_free(gpu.ctx, it.host), "nrg_host_free");

@@ -11,10 +11,7 @@
 
 namespace nrg {
 
-void die(const std::string &msg) {
-    std::cerr << "Error: " << msg << std::endl;
-    std::exit(1);
-}
+void die(const std::string &msg) { throw Error(msg); }
 
 // ---------------------------------------------------------------------------
 // param: a [param] block of keyword=value lines; duplicates are fatal,
@@ -63,6 +60,9 @@ void Params::load(const std::string &path) {
     B("broaden", broaden); B("NN2even", NN2even); B("NN2avg", NN2avg); B("NN1", NN1);
     B("lastall", lastall); B("lastalloverride", lastalloverride); B("calc0", calc0);
     B("dumpenergies", dumpenergies); D("goodE", goodE); D("NNtanh", NNtanh);
+    B("dumpsubspaces", dumpsubspaces); B("fdmexpv", fdmexpv); I("fdmexpvn", fdmexpvn); I("forcestop", forcestop); S("stopafter", stopafter);
+    I("dumpannotated", dumpannotated); B("dumpabs", dumpabs); B("dumpscaled", dumpscaled);
+    I("dumpprecision", dumpprecision); B("dumpgroups", dumpgroups); D("grouptol", grouptol);
     D("safeguard", safeguard); D("fixeps", fixeps); D("emin", emin); D("emax", emax);
     D("discard_trim", discard_trim); D("discard_immediately", discard_immediately);
     D("alpha", alpha); D("omega0", omega0); D("omega0_ratio", omega0_ratio);

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <utility>
 #include <vector>
@@ -20,6 +21,17 @@ namespace nrg {
 using Invar = std::pair<int, int>;
 using Twoinvar = std::pair<Invar, Invar>;
 
+// Errors end the solve, not the process: die() throws Error, which the entry
+// points (nrg_main, the serving workers) catch and report with status 1, the
+// exit status of the reference's my_error/exit1 (src/debug.h, portabil.h).
+struct Error : std::runtime_error {
+    explicit Error(const std::string &m) : std::runtime_error(m) {}
+};
+// A requested stop (forcestop, stopafter): the reference prints the message
+// and exits with status 1 (exit1, src/nrg.cc:3113-3114, 3453, 3462).
+struct Stop : Error {
+    explicit Stop(const std::string &m) : Error(m) {}
+};
 [[noreturn]] void die(const std::string &msg);
 
 // ---- parameters (src/param.cc defaults, src/misc.h:71-124 parser) --------
@@ -31,7 +43,15 @@ struct Params {
     int width_td = 16, prec_td = 10, width_custom = 16, prec_custom = 10, prec_xy = 10;
     bool dmnrg = false, fdm = false, dm = false, savebins = false, broaden = true, NN2even = true,
          NN2avg = false, NN1 = false, lastall = false, lastalloverride = false, calc0 = true,
-         dumpenergies = false;
+         dumpenergies = false, dumpsubspaces = false, fdmexpv = false;
+    int fdmexpvn = 0;       // iteration of the FDM expectation values (src/param.cc:428)
+    int forcestop = -1;     // stop at the start of this iteration of the first pass (src/param.cc:558)
+    std::string stopafter;  // "nrg": after the first pass, "rho": after the density matrices (src/param.cc:560-563)
+    int dumpannotated = 0;  // number of lowest states listed per iteration in annotated.dat (src/param.cc:338)
+    bool dumpabs = false, dumpscaled = true;
+    int dumpprecision = 8;
+    bool dumpgroups = true;
+    double grouptol = 1e-6;
     double goodE = 2.0, NNtanh = 0.0, safeguard = 1e-5, fixeps = 1e-15, emin = -1.0, emax = -1.0,
            discard_trim = 1e-16, discard_immediately = 1e-16, alpha = 0.3, omega0 = -1.0,
            omega0_ratio = 1.0, broaden_max = 2.5, broaden_min = -99.0, broaden_min_ratio = 3.0,
@@ -185,6 +205,7 @@ struct Options {
     bool quiet = false;  // no per-iteration log lines
     bool plain = false;  // one eigensolver call per block (the unbatched GPU path), for comparison
     double hbm_budget_gb = -1.0;  // device bytes for resident eigenvectors; <0: 60% of free HBM at start
+    std::string iter_log;         // per first-pass iteration: N, seconds since start (stream drained), eigenpairs
     DiagService *diag = nullptr;  // shared eigensolver (serving loop); null: own batched calls
 };
 

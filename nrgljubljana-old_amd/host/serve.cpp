@@ -125,6 +125,16 @@ class DiagBatcher : public DiagService {
     std::thread th_;
 };
 
+std::string json_escape(const std::string &x) {
+    std::string o;
+    for (char ch : x) {
+        if (ch == '"' || ch == '\\') o += '\\';
+        if ((unsigned char)ch < 0x20) o += ' ';
+        else o += ch;
+    }
+    return o;
+}
+
 struct Request {
     std::string id, dir;
     double arrival = 0.0;
@@ -133,6 +143,7 @@ struct Request {
     double admitted = -1.0, finished = -1.0;
     long long eigenpairs = 0;
     int rc = -1;
+    std::string error;  // message of a failed solve
 };
 
 // Device memory a solve keeps until it ends: the eigenvectors of every
@@ -235,10 +246,19 @@ int serve(const std::string &reqfile, const std::string &report, const std::stri
             o.quiet = true;
             o.stats = rp->dir + "/stats.json";
             o.diag = &batcher;
-            const int rc = run(o);
-            const long long ep = read_eigenpairs(o.stats);
+            // a failed solve is reported and the others go on (its engine
+            // has left the shared eigensolver and drained its stream)
+            int rc = 1;
+            std::string err;
+            try {
+                rc = run(o);
+            } catch (const std::exception &e) {
+                err = e.what();
+            }
+            const long long ep = rc == 0 ? read_eigenpairs(o.stats) : 0;
             std::lock_guard<std::mutex> g(m);
             rp->rc = rc;
+            rp->error = err;
             rp->eigenpairs = ep;
             rp->finished = now();
             --running;
@@ -273,7 +293,7 @@ int serve(const std::string &reqfile, const std::string &report, const std::stri
         f << (i ? ", " : "") << "{\"id\": \"" << r.id << "\", \"dir\": \"" << r.dir << "\", \"arrival\": " << r.arrival
           << ", \"admitted\": " << r.admitted << ", \"finished\": " << r.finished << ", \"latency\": "
           << r.finished - r.arrival << ", \"eigenpairs\": " << r.eigenpairs << ", \"footprint\": " << r.footprint
-          << ", \"rc\": " << r.rc << "}";
+          << ", \"rc\": " << r.rc << ", \"error\": \"" << json_escape(r.error) << "\"}";
     }
     f << "]}\n";
     return failed ? 1 : 0;
@@ -282,7 +302,7 @@ int serve(const std::string &reqfile, const std::string &report, const std::stri
 }  // namespace nrg
 
 // nrg-serve REQUESTS [--report FILE] [--slots S] [--window-ms W] [--mem-fraction F] [--tables DIR] [--device N]
-extern "C" int nrg_serve_main(int argc, char **argv) {
+static int nrg_serve_main_args(int argc, char **argv) {
     std::string req, report = "serve_report.json", tables;
     int device = 0, slots = 8;
     double window = 5.0, memf = 0.85;
@@ -310,4 +330,15 @@ extern "C" int nrg_serve_main(int argc, char **argv) {
     if (req.empty()) nrg::die("nrg-serve: no request list");
     if (tables.empty()) nrg::die("nrg-serve: --tables or NRG_TABLES required");
     return nrg::serve(req, report, tables, device, slots, window, memf);
+}
+
+// Status 0, 1 when a request failed (each is listed with its error in the
+// report) or on an error of the loop itself.
+extern "C" int nrg_serve_main(int argc, char **argv) {
+    try {
+        return nrg_serve_main_args(argc, argv);
+    } catch (const std::exception &e) {
+        std::cerr << "Error: " << e.what() << std::endl;
+        return 1;
+    }
 }

@@ -107,13 +107,18 @@ def run(out, reqs, slots=8, window_ms=5.0, mem_fraction=0.85, timeout=3600, exe=
         raise RuntimeError("bin/nrg-serve is not built; run __graft_entry__.build()")
     report = os.path.join(out, "serve_report.json")
     env = dict(os.environ, NRG_TABLES=os.path.join(PKG, "tables"))
+    # status 1 when a request failed: the report lists it with its error and
+    # the other requests are complete
     subprocess.run([exe, reqfile or os.path.join(out, "requests.txt"), "--report", report, "--slots", str(slots),
-                    "--window-ms", str(window_ms), "--mem-fraction", str(mem_fraction)], env=env, check=True,
+                    "--window-ms", str(window_ms), "--mem-fraction", str(mem_fraction)], env=env, check=False,
                    timeout=timeout)
+    if not os.path.exists(report):
+        raise RuntimeError("nrg-serve wrote no report")
     with open(report) as f:
         rep = json.load(f)
+    bad = {s["dir"] for s in rep["requests"] if s["rc"] != 0}
     for r in reqs:
-        r["zavg"] = z_average(r)
+        r["zavg"] = None if any(m in bad for m in r["members"]) else z_average(r)
     # request latency: last member finished minus arrival
     fin = {}
     for s in rep["requests"]:

@@ -54,7 +54,7 @@ DEFAULTS = dict(
     emin=-1.0, emax=-1.0, discard_trim=1e-16, discard_immediately=1e-16, alpha=0.3, omega0=-1.0,
     omega0_ratio=1.0, broaden_max=2.5, broaden_min=-99.0, broaden_min_ratio=3.0, broaden_ratio=1.05,
     width_td=16, prec_td=10, width_custom=16, prec_custom=10, prec_xy=10, Nmax=0, Ninit=0,
-    calc0=True, tdht=0,
+    calc0=True, tdht=0, dumpannotated=0, grouptol=1e-6, dumpsubspaces=False, fdmexpv=False, fdmexpvn=0,
 )
 
 
@@ -553,6 +553,8 @@ class Oracle:
             if not last:
                 f = self.recalc_f(diag, rmaxs, ancs)
             ops_s = {k: self.recalc_singlet(diag, rmaxs, ancs, v) for k, v in ops_s.items()}
+            if N == int(p["fdmexpvn"]):
+                self.ops_s_fdmexpv = ops_s
             self.custom_rows.append((sc / p["betabar"], self.expv(diag, ops_s)))
             diagprev = {I: dict(value=e["value"], shift=e["shift"]) for I, e in diag.items()}
         self.GSenergy = totalenergy
@@ -856,6 +858,85 @@ class Oracle:
             ok = (Emin < E) & (E < Emax) & (w != 0.0)
             factor = 0.5 if p["NN2avg"] else 1.0
             full.w[ok] += factor * w[ok] * win[ok]
+
+    # -------------------------------------------------- optional dump files
+    @staticmethod
+    def fcmp(x, y, eps):
+        """my_fcmp (src/numerics.h:37-53)."""
+        if x == 0.0 and y == 0.0:
+            return 0
+        if abs(x) < eps and abs(y) < eps:
+            return 0
+        if abs(x - y) / (abs(x) + abs(y)) < eps:
+            return 0
+        return 1 if x > y else -1
+
+    def annotated_levels(self):
+        """annotated.dat as data (dump_annotated, src/nrg.cc:2062-2107, called
+        from nrg_perform_measurements 2403-2409 in the first run, calc0
+        included): per iteration a list of (energy, sorted QN strings, total
+        multiplicity) groups, energies as omega_N-scaled values (dumpscaled,
+        no dumpabs)."""
+        p = self.p
+        out = []
+        per_iter = []
+        if p["calc0"]:
+            per_iter.append({I: np.asarray(v) for I, v in self.data.energies.items()})
+        per_iter += [{I: v for I, (v, k) in it.items()} for it in self.iters]
+        for vals in per_iter:
+            lst = sorted((float(e), I) for I, v in vals.items() for e in v)
+            ln = min(len(lst), int(p["dumpannotated"]))
+            i = ln
+            while i < len(lst) - 1:  # the reference never adds the last level
+                if self.fcmp(lst[i][0], lst[i - 1][0], p["grouptol"]) == 0:
+                    ln += 1
+                    i += 1
+                else:
+                    break
+            groups = []
+            i = 0
+            while i < ln:
+                i0 = i
+                q, deg = [], 0
+                while i < ln and self.fcmp(lst[i][0], lst[i0][0], p["grouptol"]) == 0:
+                    q.append("%d %d" % lst[i][1])
+                    deg += self.tab.mult(lst[i][1])
+                    i += 1
+                groups.append((lst[i0][0], sorted(q), deg))
+            out.append(groups)
+        return out
+
+    def subspaces_lines(self):
+        """subspaces.dat (dump_subspace_information, src/nrg.cc:3207-3218):
+        per iteration the subspaces with len (= kept after truncate_perform,
+        src/nrg.cc:359-367), kept and total."""
+        lines = []
+        for N in range(self.p["Ninit"], self.Nmax):
+            dm = self.dm[N]
+            lines.append("Iteration %d" % N)
+            lines.append("len_dm=%d" % len(dm))
+            for I in sorted(dm):
+                lines.append("I=%d %d len=%d kept=%d total=%d" % (I[0], I[1], dm[I]["kept"], dm[I]["kept"],
+                                                                   dm[I]["total"]))
+            lines.append("")
+        return lines
+
+    def fdmexpv_values(self):
+        """customfdm (calc_trace_fdm_kept, src/nrg.cc:973-991; evaluated at
+        iteration fdmexpvn, src/nrg.cc:2239-2247): sum_I mult(I) sum_{i,j <
+        kept} rho_FDM[I](i,j) O[I](j,i)."""
+        N = int(self.p["fdmexpvn"])
+        rho = self.rhoFDMs[N]
+        out = {}
+        for name, op in self.ops_s_fdmexpv.items():
+            tr = 0.0
+            for I, info in self.dm[N].items():
+                k = info["kept"]
+                if not k:
+                    continue
+                tr += self.tab.mult(I) * np.sum(rho[I][:k, :k] * op[(I, I)][:k, :k].T)
+            out[name] = tr
+        return out
 
     # --------------------------------------------------------------- output
     def trimmed(self, b):

@@ -123,11 +123,90 @@ def compare_with_oracle(workdir, o, spectra=(), eig_atol=1e-10, rel=1e-8):
         for x, k in zip(row, keys):
             y = ref[k]
             assert abs(x - y) <= max(1e-9 * abs(y), 5e-10), (k, x, y)  # td is printed with 10 digits
+    # custom: <O> of the singlet operators per iteration (printed with 10 digits)
+    cu = read_table(os.path.join(workdir, "custom"))
+    assert len(cu) == len(o.custom_rows)
+    for row, (Tref, vals) in zip(cu, o.custom_rows):
+        assert abs(row[0] - Tref) <= 1e-9 * abs(Tref)
+        for x, name in zip(row[1:], sorted(vals)):
+            y = vals[name]
+            assert abs(x - y) <= max(1e-9 * abs(y), 1e-12), ("custom", name, x, y)
     for kind, a, b, spin in spectra:
         name = "%s-%s" % (a, b) + {1: "-u", -1: "-d", 0: ""}[spin]
         E, W = read_bins(os.path.join(workdir, "spec_%s_dens_%s.bin" % (kind, name)))
         rE, rW = o.spectrum_bins(kind, a, b, spin)
         assert len(E) == len(rE), (kind, len(E), len(rE))
         np.testing.assert_allclose(E, rE, rtol=1e-15)
+        # every bin to `rel` of itself; the absolute floor only catches bins
+        # that are zero up to rounding
         scale = max(np.abs(rW).max(), 1e-300)
-        np.testing.assert_allclose(W, rW, rtol=rel, atol=rel * scale)
+        np.testing.assert_allclose(W, rW, rtol=rel, atol=1e-14 * scale)
+    compare_dumps(workdir, o)
+
+
+def read_annotated(path):
+    """annotated.dat: per iteration (blocks separated by an empty line) the
+    groups (energy, [QN strings], multiplicity)."""
+    its, cur = [], []
+    with open(path) as f:
+        for line in f:
+            line = line.rstrip("\n")
+            if not line.strip():
+                its.append(cur)
+                cur = []
+                continue
+            e = float(line.split()[0])
+            q = [x.strip() for x in line[line.index("(") + 1:line.rindex(")")].split(") (")]
+            deg = int(line[line.rindex("[") + 1:line.rindex("]")])
+            cur.append((e, q, deg))
+    return its
+
+
+def read_energies(path):
+    """energies.nrg / energies.dmnrg: {N: {(Q, SS): values}}."""
+    out, N, I = {}, None, None
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if line.startswith("====="):
+                N = int(line.split()[-1])
+                out[N] = {}
+            elif line.startswith("Subspace:"):
+                a, b = line.split()[1:3]
+                I = (int(a), int(b))
+            elif line:
+                out[N][I] = np.array([float(x) for x in line.split()])
+    return out
+
+
+def compare_dumps(workdir, o):
+    """annotated.dat, subspaces.dat, energies.*, customfdm when the run wrote them."""
+    p = o.p
+    path = os.path.join(workdir, "annotated.dat")
+    if int(p.get("dumpannotated", 0)) > 0:
+        got, ref = read_annotated(path), o.annotated_levels()
+        assert len(got) == len(ref)
+        for g, r in zip(got, ref):
+            assert [(q, d) for e, q, d in g] == [(q, d) for e, q, d in r]
+            for (e1, _, _), (e2, _, _) in zip(g, r):
+                assert abs(e1 - e2) <= 1e-7 * max(abs(e2), 1e-6), (e1, e2)  # 8 digits printed
+    if p.get("dumpsubspaces"):
+        with open(os.path.join(workdir, "subspaces.dat")) as f:
+            assert f.read().split("\n")[:-1] == o.subspaces_lines()
+    if p.get("dumpenergies"):
+        e1 = read_energies(os.path.join(workdir, "energies.nrg"))
+        ref = {N: {I: v for I, (v, k) in it.items()} for N, it in zip(range(p["Ninit"], o.Nmax), o.iters)}
+        for N in ref:
+            assert set(e1[N]) == set(ref[N])
+            for I in ref[N]:
+                np.testing.assert_allclose(e1[N][I], ref[N][I], rtol=1e-5, atol=1e-5)  # 6 digits printed
+        if p.get("dm"):
+            with open(os.path.join(workdir, "energies.nrg")) as f1, open(os.path.join(workdir, "energies.dmnrg")) as f2:
+                assert f1.read() == f2.read()  # the reference's check that both runs agree
+    if p.get("fdmexpv"):
+        cf = read_table(os.path.join(workdir, "customfdm"))
+        ref = o.fdmexpv_values()
+        assert len(cf) == 1 and abs(cf[0][0] - p["T"]) <= 1e-9 * p["T"]
+        for x, name in zip(cf[0][1:], sorted(ref)):
+            y = ref[name]
+            assert abs(x - y) <= max(1e-9 * abs(y), 1e-12), ("customfdm", name, x, y)

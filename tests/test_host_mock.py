@@ -19,6 +19,12 @@ CASES = {
                                                          specd="A_d-A_d", bins=50, savebins=True)),
     "qsz_fdm": dict(sym="QSZ", nch=1, Nmax=8, param=dict(keep=80, keepenergy=8.0, T=0.01, fdm=True,
                                                         specd="A_d-A_d-u A_d-A_d-d", bins=50, savebins=True)),
+    # the optional dump files: annotated.dat, subspaces.dat, energies.nrg /
+    # energies.dmnrg and the FDM expectation values (customfdm)
+    "qs_dumps": dict(sym="QS", nch=1, Nmax=9, param=dict(keep=60, keepenergy=7.0, T=0.02, fdm=True, dmnrg=True,
+                                                        specd="A_d-A_d", bins=50, savebins=True,
+                                                        dumpannotated=25, dumpsubspaces=True, dumpenergies=True,
+                                                        fdmexpv=True, fdmexpvn=2)),
     "qs2ch_fdm": dict(sym="QS", nch=2, Nmax=5, Lambda=3.0,
                       model_kw=dict(U=0.5, eps_d=-0.25, Gamma=0.02),
                       param=dict(keep=120, keepenergy=6.0, T=0.01, fdm=True, specd="A_d-A_d", bins=50,
@@ -90,3 +96,51 @@ def test_spill_to_host_changes_nothing(tmp_path, case):
     assert f0.keys() == f1.keys() and len(f0) >= 3
     for f in f0:
         assert f0[f] == f1[f], f
+
+
+def _run_rc(wd, extra=()):
+    import subprocess
+    nrgio.ensure_tables()
+    env = dict(os.environ, NRG_TABLES=nrgio.TABLES)
+    return subprocess.run([nrgio.build_mock()] + list(extra), cwd=wd, env=env, capture_output=True, text=True,
+                          timeout=300)
+
+
+def test_forcestop_stops_at_the_iteration(tmp_path):
+    """forcestop=N (src/nrg.cc:3113-3114): status 1 with the reference's
+    message at the start of iteration N; td holds the rows before it."""
+    wd = str(tmp_path)
+    nrgio.make_case(wd, Nmax=8, keep=60, keepenergy=7.0, T=0.05, fdm=True, specd="A_d-A_d", forcestop=4)
+    r = _run_rc(wd)
+    assert r.returncode == 1 and "*** Stop request at iteration 4" in r.stdout
+    td = nrgio.read_table(os.path.join(wd, "td"))
+    assert len(td) == 1 + 4  # calc0 and iterations 0..3
+    assert not os.path.exists(os.path.join(wd, "DONE"))
+    assert not any(f.startswith("spec_") for f in os.listdir(wd))
+
+
+@pytest.mark.parametrize("what,msg", [("nrg", "*** Stopped after the first sweep."),
+                                      ("rho", "*** Stopped after the DM calculation.")])
+def test_stopafter(tmp_path, what, msg):
+    """stopafter=nrg|rho (src/nrg.cc:3453, 3462): the first pass completes
+    (td, custom, subspaces.dat), no spectra, status 1."""
+    wd = str(tmp_path)
+    nrgio.make_case(wd, Nmax=7, keep=60, keepenergy=7.0, T=0.05, fdm=True, specd="A_d-A_d", stopafter=what,
+                    dumpsubspaces=True)
+    r = _run_rc(wd)
+    assert r.returncode == 1 and msg in r.stdout
+    assert len(nrgio.read_table(os.path.join(wd, "td"))) == 1 + 7
+    assert os.path.exists(os.path.join(wd, "subspaces.dat"))
+    assert not os.path.exists(os.path.join(wd, "DONE"))
+    assert not any(f.startswith("spec_") for f in os.listdir(wd))
+
+
+def test_error_returns_status_without_exit(tmp_path):
+    """An input error is reported with status 1 and the message (the
+    engine library throws; nrg_main catches)."""
+    wd = str(tmp_path)
+    nrgio.make_case(wd, Nmax=4, keep=50, T=0.1)
+    with open(os.path.join(wd, "data"), "w") as f:
+        f.write("# not a data file\n")
+    r = _run_rc(wd)
+    assert r.returncode == 1 and "Error:" in r.stderr and "version" in r.stderr

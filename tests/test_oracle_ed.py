@@ -92,6 +92,22 @@ def test_fdm_spectrum_is_exact_lehmann_without_truncation(tmp_path, sym):
     assert abs(bp.w.sum() + bn.w.sum() - 1.0) < 1e-10
 
 
+@pytest.mark.parametrize("sym", ["QS", "QSZ"])
+def test_fdm_expectation_value_is_exact_without_truncation(tmp_path, sym):
+    """customfdm (calc_trace_fdm_kept at fdmexpvn = 0): with nothing
+    discarded the FDM density matrix of shell 0 is the whole chain's thermal
+    density matrix traced down to it, so <n_d> equals ED's Tr(rho n_d) at T."""
+    Nmax = 3
+    T = 0.07
+    dg = _make(str(tmp_path), sym, 1, Nmax, T=T, fdm=True, fdmexpv=True, fdmexpvn=0)
+    o = load_oracle().Oracle(str(tmp_path)).run()
+    C, H, cd, nd = _chain(dg, 1, Nmax, Nmax=Nmax)
+    E, V = ed_chain.eig_sectors(C, H)
+    b = np.exp(-(E - E.min()) / T)
+    ref = np.einsum("ij,jk,ki->", V.T, nd, V * b[None, :]) / b.sum()
+    assert abs(o.fdmexpv_values()["n_d"] - ref) < 1e-11
+
+
 def test_chain_coefficients_closed_form():
     dg = __import__("nrgamd.datagen").datagen
     for L, N in ((2.0, 50), (3.0, 40), (4.0, 30)):

@@ -77,3 +77,22 @@ def test_z_average_is_the_mean_of_the_members(stream):
     for k in (0, len(e) // 2, len(e) - 1):
         key = np.round(e[k], 12)
         assert v[k] == pytest.approx(np.mean([l[key] for l in lut]), rel=1e-10)
+
+
+def test_failed_request_is_reported_and_the_rest_complete(tmp_path):
+    """A solve whose input is broken fails alone: the server reports it with
+    its error and finishes the other requests (no process exit)."""
+    exe = nrgio.build_mock_serve()
+    nrgio.ensure_tables()
+    out = str(tmp_path)
+    reqs = serve.make_requests(out, nreq=2, seed=5, rate=None, keep_range=(40, 60), T=1e-3, zs=(1.0,))
+    bad = reqs[1]["members"][0]
+    with open(os.path.join(bad, "data"), "w") as f:
+        f.write("# broken\n")
+    rep = serve.run(out, reqs, slots=2, exe=exe)
+    assert rep["solves"] == 2 and rep["failed"] == 1
+    by_dir = {s["dir"]: s for s in rep["requests"]}
+    assert by_dir[bad]["rc"] == 1 and "version" in by_dir[bad]["error"]
+    good = reqs[0]["members"][0]
+    assert by_dir[good]["rc"] == 0 and os.path.exists(os.path.join(good, "DONE"))
+    assert reqs[0]["zavg"] is not None and reqs[1]["zavg"] is None
