@@ -172,7 +172,10 @@ constexpr int TEAM_MAX = 32;                 // workgroups of one XCD
 constexpr int TEAM_GLOBAL_N = 1024;          // larger matrices: teams across XCDs
 constexpr int TEAM_GLOBAL_MAX = 128;
 constexpr int N_XCD = 8;                     // workgroup b runs on XCD b mod 8
-constexpr long long TEAM_SPIN_LIMIT = 1LL << 22;  // polls (~1 us each): a few seconds
+// Bounded team waits (polls of ~1 us each, about two minutes): a member may
+// first finish a long job of its own (a large blocked reduction in the same
+// launch), so the bound only ends a team that could never assemble.
+constexpr long long TEAM_SPIN_LIMIT = 1LL << 27;
 constexpr int BAR_STRIDE = 32;                    // ints between team barrier counters (128 B)
 
 struct TriJob {
@@ -798,7 +801,7 @@ constexpr int BLK_NB = 32;   // panel width
 constexpr int BLK_TWS = 65;  // row stride of a wavefront's transpose buffer (16 rows of 64)
 constexpr int BLK_TW = 16 * BLK_TWS;
 constexpr int BLK_P1W = 1 + 2 * BLK_NB;          // partial sums per block: |a|^2, W^T a, V^T a
-constexpr long long BLK_SPIN_LIMIT = 1LL << 30;  // a member may first finish a long job of its own
+constexpr long long BLK_SPIN_LIMIT = TEAM_SPIN_LIMIT;
 // dynamic LDS (doubles): 16 transpose buffers, 16 vector buffers, coefficient arrays
 constexpr int BLK_LDS_TW = 0, BLK_LDS_XS = BLK_LDS_TW + TRI_NW * BLK_TW, BLK_LDS_CF = BLK_LDS_XS + TRI_NW * 64;
 enum BlkCoef { CF_GAM = 0, CF_BET, CF_K, CF_UW, CF_UV, CF_WJ, CF_VJ1, CF_WJ1, CF_SC, CF_COUNT };

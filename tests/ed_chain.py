@@ -93,3 +93,48 @@ def lehmann(E, V, op, T):
     Om = E[:, None] - E[None, :]
     nz = W > 0
     return Om[nz], W[nz]
+
+
+def kondo_chain(J, xi, zeta, K, nch):
+    """H of the spin-1/2 Kondo impurity (J_ch S_d . s_ch(0), written
+    independently of the generator) on K+1 chain sites per channel, restricted
+    to the states with one electron on the impurity orbital (sparse operators:
+    three channels and two sites are 2^14 Fock states); returns an object
+    with N, Sz2 of the selected states and the dense restricted H."""
+    import scipy.sparse as sp
+    C = Chain(nch, K)
+
+    def cdag(mode):
+        t, s, g = C.cdag(mode)
+        return sp.csr_matrix((g, (t, s)), shape=(C.dim, C.dim))
+    cd = {m: cdag(m) for m in C.modes}
+    n = np.arange(C.dim, dtype=np.int64)
+    nd = ((n >> C.index[("d", 1)]) & 1) + ((n >> C.index[("d", -1)]) & 1)
+    sel = np.nonzero(nd == 1)[0]
+
+    def spin(o):
+        up, dn = cd[(o, 1)], cd[(o, -1)]
+        return up @ dn.T, 0.5 * (up @ up.T - dn @ dn.T)  # S+ = c_up^dag c_dn, Sz
+
+    I = sp.identity(C.dim, format="csr")
+    H = sp.csr_matrix((C.dim, C.dim))
+    spd, szd = spin("d")
+    for ch in range(nch):
+        sp0, sz0 = spin((0, ch))
+        H = H + J[ch] * (0.5 * (spd @ sp0.T + spd.T @ sp0) + szd @ sz0)
+        for s in (1, -1):
+            for k in range(K):
+                h = xi[ch][k] * cd[((k, ch), s)] @ cd[((k + 1, ch), s)].T
+                H = H + h + h.T
+        for k in range(K + 1):
+            nn = cd[((k, ch), 1)] @ cd[((k, ch), 1)].T + cd[((k, ch), -1)] @ cd[((k, ch), -1)].T
+            H = H + zeta[ch][k] * (nn - I)
+    Hs = H.tocsr()[sel][:, sel].toarray()
+
+    class Sub:
+        pass
+    R = Sub()
+    R.dim = len(sel)
+    R.N = C.N[sel]
+    R.Sz2 = C.Sz2[sel]
+    return R, Hs

@@ -104,17 +104,19 @@ def make_case(workdir, model="siam", sym="QS", nch=1, Nmax=8, Lambda=2.0, z=1.0,
     datagen.write_param(os.path.join(workdir, "param"), Lambda=Lambda, z=z, **param)
 
 
-def compare_with_oracle(workdir, o, spectra=(), eig_atol=1e-10, rel=1e-8):
-    """Engine outputs in `workdir` vs an oracle run on the same inputs."""
-    dump = read_dump(os.path.join(workdir, "eig.bin"))
-    assert len(dump) == len(o.iters)
-    for (N, it), ref in zip(dump, o.iters):
-        assert set(it) == set(ref), ("subspace lists differ at N=%d" % N)
-        for I, (v, k) in it.items():
-            rv, rk = ref[I]
-            assert len(v) == len(rv)
-            assert k == rk, ("kept count differs", N, I, k, rk)
-            np.testing.assert_allclose(v, rv, atol=eig_atol, rtol=0)
+def compare_with_oracle(workdir, o, spectra=(), eig_atol=1e-10, rel=1e-8, eigen=True):
+    """Engine outputs in `workdir` vs an oracle run on the same inputs
+    (eigen=False: a run without the --dump-eigen file, e.g. a served solve)."""
+    if eigen:
+        dump = read_dump(os.path.join(workdir, "eig.bin"))
+        assert len(dump) == len(o.iters)
+        for (N, it), ref in zip(dump, o.iters):
+            assert set(it) == set(ref), ("subspace lists differ at N=%d" % N)
+            for I, (v, k) in it.items():
+                rv, rk = ref[I]
+                assert len(v) == len(rv)
+                assert k == rk, ("kept count differs", N, I, k, rk)
+                np.testing.assert_allclose(v, rv, atol=eig_atol, rtol=0)
     td = read_table(os.path.join(workdir, "td"))
     assert len(td) == len(o.td_rows)
     keys = ["T", "<Sz^2>", "<Q>", "<Q^2>", "E", "E2", "C", "F", "S"] if o.sym == "QS" else \

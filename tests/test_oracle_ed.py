@@ -108,6 +108,34 @@ def test_fdm_expectation_value_is_exact_without_truncation(tmp_path, sym):
     assert abs(o.fdmexpv_values()["n_d"] - ref) < 1e-11
 
 
+def _kondo_case(tmp, nch, Nmax, J=0.4, Lambda=4.0):
+    dg = load_nrgamd().datagen if hasattr(load_nrgamd(), "datagen") else __import__("nrgamd.datagen").datagen
+    model = dg.kondo(sym="QSZ", nch=nch, J=J, Lambda=Lambda)
+    model.write(os.path.join(tmp, "data"), Lambda, 1.0, Nmax)
+    dg.write_param(os.path.join(tmp, "param"), Lambda=Lambda, keep=100000, T=0.3, Nmax=Nmax)
+    return dg
+
+
+@pytest.mark.parametrize("nch,Nmax", [(1, 3), (3, 1)])
+def test_kondo_eigenvalues_match_ed(tmp_path, nch, Nmax):
+    """The Kondo generator (models.m:382-394: J S_d . s_0 per channel, the
+    local moment carrying no charge) and the QSZ tables: NRG eigenvalues equal
+    ED of the whole chain, one channel up to 5 sites, three channels at the
+    first iteration (8192 states, the three-channel configuration's first
+    shell)."""
+    J, Lambda = 0.4, 4.0
+    dg = _kondo_case(str(tmp_path), nch, Nmax, J=J, Lambda=Lambda)
+    o = load_oracle().Oracle(str(tmp_path)).run()
+    xi, zeta = dg.chain_coefficients(Lambda, 1.0, Nmax)
+    for N in range(Nmax):
+        C, H = ed_chain.kondo_chain([J] * nch, [xi] * nch, [zeta] * nch, N + 1, nch)
+        E, V = ed_chain.eig_sectors(C, H)
+        ref = np.sort((E - E.min()) / o.SCALE(N + 1))
+        mine = np.sort(np.concatenate([np.repeat(v, o.tab.mult(I)) for I, (v, k) in o.iters[N].items()]))
+        assert len(mine) == len(ref)
+        np.testing.assert_allclose(mine, ref, atol=1e-10, rtol=0)
+
+
 def test_chain_coefficients_closed_form():
     dg = __import__("nrgamd.datagen").datagen
     for L, N in ((2.0, 50), (3.0, 40), (4.0, 30)):
