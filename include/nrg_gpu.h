@@ -184,6 +184,21 @@ typedef struct {
 nrg_status nrg_gemm_grouped(nrg_ctx *ctx, const nrg_gemm_out *outs, int nouts,
                             const nrg_gemm_term *terms, int nterms);
 
+/* No counterpart in the reference.  nrg_gemm_grouped for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same kernels and results, with the
+ * launch's descriptors and the partial tiles of split outputs in a fixed
+ * device array of NRG_NOALLOC_SCRATCH_BYTES per device instead of the
+ * context's staging ring and partial-tile buffer.  The caller passes inputs
+ * and outputs as for nrg_gemm_grouped; no hipMalloc, hipFree or other
+ * allocation is made.  A launch needing more scratch returns NRG_ERR_ARG
+ * with the sizes in nrg_last_error.  Every call on one device shares that
+ * device's array: calls on one context are ordered by its stream, calls that
+ * overlap on one device from different contexts race. */
+#define NRG_NOALLOC_SCRATCH_BYTES (16u << 20)
+nrg_status nrg_gemm_grouped_noalloc(nrg_ctx *ctx, const nrg_gemm_out *outs, int nouts,
+                                    const nrg_gemm_term *terms, int nterms);
+
 /* ---- spectral weights + binning ----------------------------------------- */
 /* Replaces the delta-peak loops of SPEC_FDMgt::calc / SPEC_FDMls::calc
  * (src/spec_FDM.cc:41-119) and SPEC_DMNRG::calc (src/spec_DMNRG.cc:13-52)

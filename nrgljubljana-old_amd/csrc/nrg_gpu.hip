@@ -616,7 +616,8 @@ __global__ __launch_bounds__(256) void k_spec_bin(const nrg_spec_params P, const
 
 nrg_status gemm_launch_dev(nrg_ctx *c, int kid, const nrg_gemm_out *d_outs, const nrg_gemm_term *d_terms,
                            const GemmItem *d_items, int nitems, const GemmItem *d_reduce, int nreduce,
-                           double flops, double bytes, bool mfma) {
+                           double flops, double bytes, bool mfma, double *part) {
+    if (!part) part = c->gpart;
     if (nitems <= 0) return NRG_OK;
     LaunchTimer lt(kid, c->stream, flops, bytes);
     if (mfma) {
@@ -624,9 +625,9 @@ nrg_status gemm_launch_dev(nrg_ctx *c, int kid, const nrg_gemm_out *d_outs, cons
         HIPCHK(c, hipGetLastError());
         return NRG_OK;
     }
-    hipLaunchKernelGGL(k_gemm, dim3(nitems), dim3(256), 0, c->stream, d_outs, d_terms, d_items, c->gpart);
+    hipLaunchKernelGGL(k_gemm, dim3(nitems), dim3(256), 0, c->stream, d_outs, d_terms, d_items, part);
     if (nreduce > 0)
-        hipLaunchKernelGGL(k_gemm_reduce, dim3(nreduce), dim3(256), 0, c->stream, d_outs, d_reduce, c->gpart);
+        hipLaunchKernelGGL(k_gemm_reduce, dim3(nreduce), dim3(256), 0, c->stream, d_outs, d_reduce, part);
     HIPCHK(c, hipGetLastError());
     return NRG_OK;
 }
@@ -635,6 +636,11 @@ nrg_status gemm_launch_dev(nrg_ctx *c, int kid, const nrg_gemm_out *d_outs, cons
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
+
+// Scratch of nrg_gemm_grouped_noalloc: one array per device, held until the
+// process ends; host code uses the address hipGetSymbolAddress returns for
+// the context's device (c->scratch), never the array's name.
+static __device__ __align__(256) double scratch[NRG_NOALLOC_SCRATCH_BYTES / sizeof(double)];
 
 nrg_status nrg_ctx_create(nrg_ctx **out, int device) {
     if (!out) return NRG_ERR_ARG;
@@ -649,6 +655,7 @@ nrg_status nrg_ctx_create(nrg_ctx **out, int device) {
     if (e == hipSuccess) e = hipMalloc(&c->dstatus, sizeof(int));
     if (e == hipSuccess) e = hipHostMalloc((void **)&c->hstat, sizeof(int), hipHostMallocDefault);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_values, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipGetSymbolAddress((void **)&c->scratch, HIP_SYMBOL(scratch));
     for (int k = 0; k < nrg_ctx::NSTAGE && e == hipSuccess; ++k)
         e = hipEventCreateWithFlags(&c->desc_ev[k], hipEventDisableTiming);
     if (e != hipSuccess) {
@@ -888,9 +895,18 @@ nrg_status nrg_assemble_batch(nrg_ctx *c, const nrg_asm_block *blocks, int nbloc
     return NRG_OK;
 }
 
-nrg_status nrg_gemm_grouped(nrg_ctx *c, const nrg_gemm_out *outs, int nouts, const nrg_gemm_term *terms,
-                            int nterms) {
-    if (nouts <= 0) return NRG_OK;
+// Checks the descriptors and builds one grouped launch: the packed
+// descriptors (outputs, terms, work items, reduce items) in `buf`, their
+// offsets, the kernel choice and the partial slots split tiles need.
+struct GemmLaunch {
+    std::vector<char> buf;
+    size_t ob = 0, tb = 0, ib = 0, rb = 0;
+    int nitems = 0, nreduce = 0, slots = 0;
+    bool mfma = false;
+    double flops = 0, bytes = 0;
+};
+static nrg_status gemm_prepare(nrg_ctx *c, const nrg_gemm_out *outs, int nouts, const nrg_gemm_term *terms,
+                               int nterms, GemmLaunch &L) {
     std::vector<GemmItem> items;
     for (int o = 0; o < nouts; ++o) {
         const nrg_gemm_out &O = outs[o];
@@ -903,35 +919,72 @@ nrg_status nrg_gemm_grouped(nrg_ctx *c, const nrg_gemm_out *outs, int nouts, con
         }
     }
     std::vector<GemmItem> reduce;
-    int slots = 0;
-    const bool mfma = gemm_plan(std::vector<nrg_gemm_out>(outs, outs + nouts), 0, gemm_ksteps(outs, nouts, terms),
-                                items, reduce, slots);
-    if (items.empty()) return NRG_OK;
-    if (slots) {
-        nrg_status s = gemm_reserve(c, slots);
-        if (s) return s;
-    }
-    const size_t ob = sizeof(nrg_gemm_out) * nouts, tb = sizeof(nrg_gemm_term) * nterms,
-                 ib = sizeof(GemmItem) * items.size(), rb = sizeof(GemmItem) * reduce.size();
-    std::vector<char> buf(ob + tb + ib + rb);
-    memcpy(buf.data(), outs, ob);
-    if (tb) memcpy(buf.data() + ob, terms, tb);
-    memcpy(buf.data() + ob + tb, items.data(), ib);
-    if (rb) memcpy(buf.data() + ob + tb + ib, reduce.data(), rb);
-    void *d;
-    nrg_status s = stage(c, buf.data(), buf.size(), &d);
-    if (s) return s;
-    double gflops = 0, gbytes = 0;
+    L.mfma = gemm_plan(std::vector<nrg_gemm_out>(outs, outs + nouts), 0, gemm_ksteps(outs, nouts, terms), items,
+                       reduce, L.slots);
+    L.ob = sizeof(nrg_gemm_out) * nouts;
+    L.tb = sizeof(nrg_gemm_term) * nterms;
+    L.ib = sizeof(GemmItem) * items.size();
+    L.rb = sizeof(GemmItem) * reduce.size();
+    L.nitems = (int)items.size();
+    L.nreduce = (int)reduce.size();
+    L.buf.resize(L.ob + L.tb + L.ib + L.rb);
+    memcpy(L.buf.data(), outs, L.ob);
+    if (L.tb) memcpy(L.buf.data() + L.ob, terms, L.tb);
+    if (L.ib) memcpy(L.buf.data() + L.ob + L.tb, items.data(), L.ib);
+    if (L.rb) memcpy(L.buf.data() + L.ob + L.tb + L.ib, reduce.data(), L.rb);
     for (int o = 0; o < nouts; ++o)
         for (int t = outs[o].term_begin; t < outs[o].term_begin + outs[o].term_count; ++t) {
             const double m = outs[o].m, n = outs[o].n, k = terms[t].k;
-            gflops += 2.0 * m * n * k;
-            gbytes += 8.0 * (m * k + k * n);
+            L.flops += 2.0 * m * n * k;
+            L.bytes += 8.0 * (m * k + k * n);
         }
-    for (int o = 0; o < nouts; ++o) gbytes += 8.0 * outs[o].m * outs[o].n * (outs[o].beta != 0.0 ? 2 : 1);
-    return gemm_launch_dev(c, K_GEMM, (const nrg_gemm_out *)d, (const nrg_gemm_term *)((char *)d + ob),
-                           (const GemmItem *)((char *)d + ob + tb), (int)items.size(),
-                           (const GemmItem *)((char *)d + ob + tb + ib), (int)reduce.size(), gflops, gbytes, mfma);
+    for (int o = 0; o < nouts; ++o) L.bytes += 8.0 * outs[o].m * outs[o].n * (outs[o].beta != 0.0 ? 2 : 1);
+    return NRG_OK;
+}
+
+static nrg_status gemm_launch_staged(nrg_ctx *c, const GemmLaunch &L, const char *d, double *part) {
+    return gemm_launch_dev(c, K_GEMM, (const nrg_gemm_out *)d, (const nrg_gemm_term *)(d + L.ob),
+                           (const GemmItem *)(d + L.ob + L.tb), L.nitems, (const GemmItem *)(d + L.ob + L.tb + L.ib),
+                           L.nreduce, L.flops, L.bytes, L.mfma, part);
+}
+
+nrg_status nrg_gemm_grouped(nrg_ctx *c, const nrg_gemm_out *outs, int nouts, const nrg_gemm_term *terms,
+                            int nterms) {
+    if (nouts <= 0) return NRG_OK;
+    GemmLaunch L;
+    nrg_status s = gemm_prepare(c, outs, nouts, terms, nterms, L);
+    if (s) return s;
+    if (!L.nitems) return NRG_OK;
+    if (L.slots) {
+        s = gemm_reserve(c, L.slots);
+        if (s) return s;
+    }
+    void *d;
+    s = stage(c, L.buf.data(), L.buf.size(), &d);
+    if (s) return s;
+    return gemm_launch_staged(c, L, (const char *)d, c->gpart);
+}
+
+nrg_status nrg_gemm_grouped_noalloc(nrg_ctx *c, const nrg_gemm_out *outs, int nouts, const nrg_gemm_term *terms,
+                                    int nterms) {
+    if (!c || !c->scratch) return fail(c, NRG_ERR_ARG, "gemm_noalloc: no device scratch array on this context");
+    if (nouts <= 0) return NRG_OK;
+    GemmLaunch L;
+    nrg_status s = gemm_prepare(c, outs, nouts, terms, nterms, L);
+    if (s) return s;
+    if (!L.nitems) return NRG_OK;
+    // [descriptors | partial tiles of split outputs] in the fixed array
+    const size_t desc = (L.buf.size() + 255) & ~(size_t)255;
+    const size_t need = desc + (size_t)L.slots * GT * GT * sizeof(double);
+    if (need > NRG_NOALLOC_SCRATCH_BYTES)
+        return fail(c, NRG_ERR_ARG, "gemm_noalloc: the launch needs " + std::to_string(need) +
+                                        " bytes of scratch, the fixed device array holds " +
+                                        std::to_string((size_t)NRG_NOALLOC_SCRATCH_BYTES));
+    char *base = (char *)c->scratch;
+    // a copy on the context's stream: ordered after the work of the previous
+    // call that still reads the array
+    HIPCHK(c, hipMemcpyAsync(base, L.buf.data(), L.buf.size(), hipMemcpyHostToDevice, c->stream));
+    return gemm_launch_staged(c, L, base, (double *)(base + desc));
 }
 
 nrg_status nrg_spec_bin(nrg_ctx *c, const nrg_spec_params *prm, const nrg_spec_pair *pairs, int npairs,

@@ -63,6 +63,9 @@ struct nrg_ctx {
     size_t values_count = 0;
     double *hpin = nullptr;
     size_t hpin_doubles = 0;
+    // device address of the no-allocation scratch array on this context's
+    // device (hipGetSymbolAddress at creation)
+    void *scratch = nullptr;
 };
 
 inline nrg_status fail(nrg_ctx *c, nrg_status code, const std::string &msg) {
@@ -245,7 +248,7 @@ nrg_status gemm_reserve(nrg_ctx *c, int slots);
 // tiles past the patched m exit), then the reduction of split tiles.
 nrg_status gemm_launch_dev(nrg_ctx *c, int kid, const nrg_gemm_out *d_outs, const nrg_gemm_term *d_terms,
                            const GemmItem *d_items, int nitems, const GemmItem *d_reduce, int nreduce,
-                           double flops, double bytes, bool mfma = false);
+                           double flops, double bytes, bool mfma = false, double *part = nullptr);
 // Launches with at least GEMM_MFMA_MIN_TILES tiles of GTM x GTM run on FP64
 // MFMA (k_gemm_mfma, tiles never split); smaller ones on the 32 x 32 FMA tile
 // (k_gemm, split when few).  gemm_plan appends the work items of the outputs

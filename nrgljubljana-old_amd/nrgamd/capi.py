@@ -14,7 +14,7 @@ LIB = os.path.join(PKG, "lib", "libnrg_gpu.so")
 
 EXPORTS = ["nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_sync", "nrg_dev_alloc", "nrg_dev_free",
            "nrg_h2d", "nrg_d2h", "nrg_d2h_strided", "nrg_zero", "nrg_gather_d2h", "nrg_gather_dev", "nrg_scatter_h2d", "nrg_mem_info", "nrg_host_alloc", "nrg_host_free", "nrg_assemble_batch", "nrg_diag_batch", "nrg_diag_values",
-           "nrg_gemm_grouped", "nrg_spec_bin", "nrg_timing_enable", "nrg_timing_filter", "nrg_timing_reset",
+           "nrg_gemm_grouped", "nrg_gemm_grouped_noalloc", "nrg_spec_bin", "nrg_timing_enable", "nrg_timing_filter", "nrg_timing_reset",
            "nrg_timing_get"]
 
 
@@ -145,6 +145,13 @@ class Ctx:
         o = (GemmOut * len(outs))(*outs)
         t = (GemmTerm * max(1, len(terms)))(*terms)
         self.check(self.lib.nrg_gemm_grouped(self.ctx, o, len(outs), t, len(terms)), "gemm_grouped")
+
+    def gemm_grouped_noalloc(self, outs, terms):
+        """The no-allocation variant (fixed device scratch array); returns the
+        status instead of raising, so that callers can test its limit."""
+        o = (GemmOut * len(outs))(*outs)
+        t = (GemmTerm * max(1, len(terms)))(*terms)
+        return self.lib.nrg_gemm_grouped_noalloc(self.ctx, o, len(outs), t, len(terms))
 
     def assemble(self, blocks, terms):
         b = (AsmBlock * len(blocks))(*blocks)
