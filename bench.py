@@ -84,8 +84,9 @@ def ensure_tables():
 
 def run_engine(lib, wd, args):
     """One solve in-process: chdir into the work directory and call nrg_main
-    (its log goes to stderr, so stdout keeps the one JSON line)."""
-    argv_l = [b"nrg", b"--quiet", b"--tables", os.path.join(PKG, "tables").encode()] + [a.encode() for a in args]
+    (its log, a line per iteration, goes to stderr, so stdout keeps the one
+    JSON line)."""
+    argv_l = [b"nrg", b"--tables", os.path.join(PKG, "tables").encode()] + [a.encode() for a in args]
     argv = (ctypes.c_char_p * len(argv_l))(*argv_l)
     cwd = os.getcwd()
     os.chdir(wd)
