@@ -333,31 +333,43 @@ class Oracle:
         return h, off
 
     # ------------------------------------------------------------ truncation
-    def fix_splittings(self, diag):
-        energies = np.sort(np.concatenate([e["value"] for e in diag.values()]))
+    def find_clusters(self, energies):
+        """find_clusters (src/splitting.cc:44-77) over ascending energies: a
+        cluster runs while values stay less than fixeps above its first value;
+        when a later value closes it and its members are not all equal
+        (cluster_splitting), every member unlike the first maps to the first.
+        The last cluster is never closed.  Entries go into the run-wide
+        STAT::cluster_mapping (src/nrg.cc:561), an existing key is kept
+        (std::map::insert)."""
         eps = self.p["fixeps"]
-        mapping = {}
+        mapping = self.__dict__.setdefault("cluster_mapping", {})
         e0 = energies[0]
         i0 = 0
-        size = 1
-        for i in range(len(energies) + 1):
-            if i < len(energies) and (energies[i] - e0) < eps:
-                size += 1
+        for i in range(len(energies)):
+            if (energies[i] - e0) < eps:
                 continue
-            i1 = i
-            if size > 1:
-                cl = energies[i0:i1]
-                if np.any(cl != cl[0]):
-                    for x in cl[1:]:
-                        if x != cl[0]:
-                            mapping[x] = cl[0]
-            if i < len(energies):
-                e0 = energies[i]
-                i0 = i
-                size = 1
-        if mapping:
-            for e in diag.values():
-                e["value"] = np.array([mapping.get(x, x) for x in e["value"]])
+            cl = energies[i0:i]
+            for x in cl[1:]:
+                if x != cl[0]:
+                    mapping.setdefault(float(x), float(cl[0]))
+            e0 = energies[i]
+            i0 = i
+
+    def fix_splittings(self, diag):
+        """The sequence of src/nrg.cc:2910-2918: clusters of the sorted
+        energies into the mapping, every value found in the mapping replaced
+        (fix_splittings, src/splitting.cc:5-17), and after a replacement the
+        clusters of the new values added too."""
+        self.find_clusters(np.sort(np.concatenate([e["value"] for e in diag.values()])))
+        mapping = self.cluster_mapping
+        changed = False
+        for e in diag.values():
+            new = np.array([mapping.get(float(x), x) for x in e["value"]]) if mapping else e["value"]
+            if mapping and np.any(new != e["value"]):
+                changed = True
+                e["value"] = new
+        if changed:
+            self.find_clusters(np.sort(np.concatenate([e["value"] for e in diag.values()])))
 
     def truncation(self, diag, last, dmnrgrun):
         p = self.p
@@ -967,6 +979,54 @@ class Oracle:
         ep, wp = self.trimmed(spos)
         en, wn = self.trimmed(sneg)
         return np.concatenate([ep, -en]), np.concatenate([wp, wn])
+
+    def broadening_mesh(self):
+        """make_mesh + get_broaden_min (src/spectrumrealfreq.cc:212-229):
+        broaden_max, divided by broaden_ratio while above the lowest energy
+        (broaden_min, else broaden_min_ratio x SCALE(Nmax))."""
+        p = self.p
+        bmin = p["broaden_min"] if p["broaden_min"] > 0.0 else p["broaden_min_ratio"] * self.SCALE(self.Nmax)
+        mesh = []
+        E = p["broaden_max"]
+        while E > bmin:
+            mesh.append(E)
+            E /= p["broaden_ratio"]
+        return np.array(mesh)
+
+    def br_new(self, e, ept):
+        """BR_NEW (src/spectral.h:56-102) for arrays: the log-Gaussian
+        BR_L(e, ept) = exp(-(ln(e/ept)/alpha - alpha/4)^2) / (alpha |e| sqrt(pi))
+        (zero across the sign change and at ept = 0), blended below |e| = omega0
+        into the Gaussian BR_G of width omega0 with h = exp(-(ln(|e|/omega0)/alpha)^2)."""
+        p = self.p
+        alpha = p["alpha"]
+        gamma = alpha / 4.0
+        omega0 = p["omega0_ratio"] * p["T"] if p["omega0"] < 0.0 else p["omega0"]
+        e, ept = np.broadcast_arrays(np.asarray(e, float), np.asarray(ept, float))
+        same = (e * ept) > 0.0
+        with np.errstate(divide="ignore", invalid="ignore"):
+            x = np.where(same, np.log(np.where(same, e / ept, 1.0)) / alpha - gamma, 0.0)
+            bl = np.where(same, np.exp(-x * x) / (alpha * np.abs(e) * math.sqrt(math.pi)), 0.0)
+            y = np.log(np.abs(e) / omega0) / alpha
+            h = np.exp(-y * y)
+            bg = np.exp(-((e - ept) / omega0) ** 2) / (omega0 * math.sqrt(math.pi))
+        return np.where(np.abs(e) > omega0, bl, bl * h + bg * (1.0 - h))
+
+    def broadened(self, kind, a, b, spin=0):
+        """SpectrumRealFreq::continuous (src/spectrumrealfreq.cc:271-306): the
+        trimmed bins of both signs broadened onto the mesh at +E and -E;
+        returns (E, A) in the order spec_<kind>_dens_<a>-<b>.dat lists them
+        (negative energies ascending, then positive ascending)."""
+        spos, sneg = self.spectra[(kind, a, b, spin)]
+        ep, wp = self.trimmed(spos)
+        en, wn = self.trimmed(sneg)
+        mesh = self.broadening_mesh()
+        M = mesh[:, None]
+        vpos = (wp[None, :] * self.br_new(M, ep[None, :])).sum(1) + (wn[None, :] * self.br_new(M, -en[None, :])).sum(1)
+        vneg = (wp[None, :] * self.br_new(-M, ep[None, :])).sum(1) + (wn[None, :] * self.br_new(-M, -en[None, :])).sum(1)
+        order = np.argsort(mesh)
+        return (np.concatenate([-mesh[order][::-1], mesh[order]]),
+                np.concatenate([vneg[order][::-1], vpos[order]]))
 
 
 def run(workdir):

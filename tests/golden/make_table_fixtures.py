@@ -12,6 +12,7 @@ against them.
 Run in a container where the reference tree is mounted:
     python tests/golden/make_table_fixtures.py /root/reference
 """
+import ast
 import json
 import math
 import os
@@ -36,11 +37,35 @@ def _sqrt(x):
     return math.sqrt(x)
 
 
+_FUNCS = {"S": S, "Power": Power, "sqrt": _sqrt}
+_BINOPS = {ast.Add: lambda a, b: a + b, ast.Sub: lambda a, b: a - b, ast.Mult: lambda a, b: a * b,
+           ast.Div: lambda a, b: a / b, ast.Pow: lambda a, b: a ** b}
+
+
+def _eval_node(node, env):
+    """Arithmetic over numbers, the variables in `env`, + - * / ** and calls
+    of S / Power / sqrt; any other syntax is refused (the expression text
+    comes from an untrusted tree and is never executed)."""
+    if isinstance(node, ast.Expression):
+        return _eval_node(node.body, env)
+    if isinstance(node, ast.Constant) and type(node.value) in (int, float):
+        return node.value
+    if isinstance(node, ast.Name) and node.id in env:
+        return env[node.id]
+    if isinstance(node, ast.BinOp) and type(node.op) in _BINOPS:
+        return _BINOPS[type(node.op)](_eval_node(node.left, env), _eval_node(node.right, env))
+    if isinstance(node, ast.UnaryOp) and isinstance(node.op, (ast.USub, ast.UAdd)):
+        v = _eval_node(node.operand, env)
+        return -v if isinstance(node.op, ast.USub) else v
+    if (isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and node.func.id in _FUNCS
+            and not node.keywords):
+        return _FUNCS[node.func.id](*[_eval_node(a, env) for a in node.args])
+    raise ValueError("unsupported expression element: %s" % ast.dump(node)[:80])
+
+
 def evaluate(expr, env):
-    glb = {"S": S, "Power": Power, "sqrt": _sqrt, "__builtins__": {}}
     try:
-        v = eval(expr, glb, dict(env))
-        v = float(v)
+        v = float(_eval_node(ast.parse(expr, mode="eval"), env))
         if not math.isfinite(v):
             return None
         return v

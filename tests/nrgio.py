@@ -143,6 +143,13 @@ def compare_with_oracle(workdir, o, spectra=(), eig_atol=1e-10, rel=1e-8, eigen=
         # that are zero up to rounding
         scale = max(np.abs(rW).max(), 1e-300)
         np.testing.assert_allclose(W, rW, rtol=rel, atol=1e-14 * scale)
+        if o.p["broaden"]:
+            # the broadened curve (.dat, printed with prec_xy = 10 digits)
+            dat = np.loadtxt(os.path.join(workdir, "spec_%s_dens_%s.dat" % (kind, name)), ndmin=2)
+            rE, rA = o.broadened(kind, a, b, spin)
+            assert dat.shape == (len(rE), 2), (kind, dat.shape, len(rE))
+            np.testing.assert_allclose(dat[:, 0], rE, rtol=1e-9)
+            np.testing.assert_allclose(dat[:, 1], rA, rtol=rel + 1e-9, atol=1e-14 * max(np.abs(rA).max(), 1e-300))
     compare_dumps(workdir, o)
 
 

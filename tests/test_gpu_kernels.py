@@ -90,8 +90,10 @@ def test_diag_batch_degenerate_and_structured(ctx):
 
 
 def test_diag_batch_large(ctx):
-    # 1345: tridiagonalisation vectors in global memory; 2100: top merge of
-    # the divide and conquer above the LDS size of the deflation kernel
+    # 700: the streaming team tridiagonalisation; 1100, 1345, 2100: the
+    # blocked one (from 1024 rows); the divide-and-conquer merges of all four
+    # deflate in LDS (the global-memory deflation above 5000 rows is covered
+    # by the 7000-row case of the blocked tests)
     rng = np.random.default_rng(3)
     mats = [_sym(rng, 700), _sym(rng, 1100), _sym(rng, 1345), _sym(rng, 2100)]
     out = ctx.diag_batch(mats)
