@@ -20,6 +20,7 @@
 #include <set>
 #include <sstream>
 #include <thread>
+#include <unordered_map>
 
 #include <mutex>
 
@@ -461,11 +462,28 @@ struct Engine {
     }
 
     // ------------------------------------------------------------ truncation
-    // find_clusters + fix_splittings (src/splitting.cc:5-77): in ascending
-    // order, a cluster starts at the first value at least fixeps above the
-    // current cluster's first value, and every member takes that first value.
-    // One sort of (value, location); the fixed values, still ascending, are
-    // kept in it.sorted for the truncation.
+    // STAT::cluster_mapping (src/nrg.cc:561): value -> replacement, kept over
+    // every iteration of both passes; an entry, once made, is never replaced.
+    std::unordered_map<double, double> cluster_map;
+    // find_clusters (src/splitting.cc:44-77) over ascending values: a cluster
+    // runs while values stay less than fixeps above its first value; when a
+    // later value closes it and its members are not all equal, every member
+    // unlike the first maps to the first.  The last cluster is never closed.
+    void find_clusters(const std::vector<KeyedRef> &e) {
+        size_t i0 = 0;
+        for (size_t i = 0; i < e.size(); ++i) {
+            if (*e[i].ref - *e[i0].ref < P.fixeps) continue;
+            for (size_t k = i0 + 1; k < i; ++k)
+                if (*e[k].ref != *e[i0].ref) cluster_map.emplace(*e[k].ref, *e[i0].ref);
+            i0 = i;
+        }
+    }
+    // find_clusters + fix_splittings (src/splitting.cc:5-17; the sequence of
+    // src/nrg.cc:2910-2918): clusters of this iteration's sorted values are
+    // added to the map, every value found in the map is replaced, and after
+    // a replacement the clusters of the new values are added as well.  One
+    // radix sort of (value, location) (two after a replacement); the fixed
+    // values, ascending, are kept in it.sorted for the truncation.
     std::vector<KeyedRef> sort_buf, sort_tmp;
     void fix_splittings(Iter &it) {
         std::vector<KeyedRef> &e = sort_buf;
@@ -474,14 +492,22 @@ struct Engine {
             for (double &x : kv.second.value) e.push_back({order_key(x), &x});
         if (e.empty()) return;
         radix_sort(e, sort_tmp);
-        it.sorted.resize(e.size());
-        double e0 = *e[0].ref;  // first value of the current cluster
-        for (size_t i = 0; i < e.size(); ++i) {
-            const double x = *e[i].ref;
-            if (!(x - e0 < P.fixeps)) e0 = x;
-            *e[i].ref = e0;
-            it.sorted[i] = e0;
+        find_clusters(e);
+        bool changed = false;
+        if (!cluster_map.empty())
+            for (KeyedRef &k : e) {
+                auto m = cluster_map.find(*k.ref);
+                if (m == cluster_map.end()) continue;
+                *k.ref = m->second;
+                k.key = order_key(m->second);
+                changed = true;
+            }
+        if (changed) {
+            radix_sort(e, sort_tmp);
+            find_clusters(e);
         }
+        it.sorted.resize(e.size());
+        for (size_t i = 0; i < e.size(); ++i) it.sorted[i] = *e[i].ref;
     }
 
     // highest_retained_energy + nrg_truncate_prepare (src/nrg.cc:1919-1998)

@@ -25,6 +25,10 @@ CASES = {
                                                         specd="A_d-A_d", bins=50, savebins=True,
                                                         dumpannotated=25, dumpsubspaces=True, dumpenergies=True,
                                                         fdmexpv=True, fdmexpvn=2)),
+    # cluster snapping with a coarse fixeps: many clusters snap, the mapping
+    # carries over iterations and passes, the top cluster stays open
+    "qs_fixeps": dict(sym="QS", nch=1, Nmax=10, param=dict(keep=80, keepenergy=8.0, T=0.01, fdm=True, dmnrg=True,
+                                                          specd="A_d-A_d", bins=50, savebins=True, fixeps=2e-3)),
     "qs2ch_fdm": dict(sym="QS", nch=2, Nmax=5, Lambda=3.0,
                       model_kw=dict(U=0.5, eps_d=-0.25, Gamma=0.02),
                       param=dict(keep=120, keepenergy=6.0, T=0.01, fdm=True, specd="A_d-A_d", bins=50,
