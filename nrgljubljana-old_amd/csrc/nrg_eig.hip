@@ -195,6 +195,22 @@ __device__ inline double ld_dev(const double *p) {
     return __hip_atomic_load((gdouble *)const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Memory-model assumption of the team kernels.  Everything one workgroup
+// hands to another goes through agent-scope atomic stores and loads
+// (st_dev / ld_dev: sc1 accesses, which bypass the non-coherent vector and
+// L2 caches on gfx950), and the team barrier orders them with s_waitcnt
+// vmcnt(0) before a relaxed counter increment and __syncthreads after the
+// poll: the hardware completes an acknowledged agent-scope store before any
+// later agent-scope load of another CU can miss it.  The C++/HIP model
+// would ask for release on the increment and acquire on the poll; on
+// gfx950 those add an L2 writeback (buffer_wbl2 sc1) per arrival and an L2
+// invalidate (buffer_inv sc1) per exit, measured at +2 % (one 13136-row
+// matrix) to +4 % (the keep-10000 blocks) of the blocked reduction.
+// Checked with ROCm 7.2 (AMD clang 22); the build refuses other versions
+// until the assumption has been checked there.
+static_assert(HIP_VERSION_MAJOR == 7 && HIP_VERSION_MINOR >= 2,
+              "team barriers rely on gfx950 agent-scope atomic coherence as checked with ROCm 7.2");
+
 // Team barrier with a bounded wait: returns false (and flags the status) if
 // the team did not arrive, so that every workgroup still terminates.
 __device__ inline bool team_barrier(int *cnt, int target, int *status, int *s_flag,
