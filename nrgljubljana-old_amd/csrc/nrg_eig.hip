@@ -829,10 +829,12 @@ static_assert(BLK_NB <= 64 && 1 + 2 * BLK_NB <= 16 * 64, "coefficient arrays");
 
 // S1 layout of a blocked reduction (doubles from a 512-byte aligned start):
 // V and p of the panel (BLK_NB x npad each), the column, partial sums (P1,
-// P3), row and column partial products of the tiles (nt x nt x 64 each).
+// P3), row partial products of the tiles (nt x nt x 64), and per block the
+// column partial products summed by each wavefront of its owner
+// (nt x TRI_NW x 64).
 __host__ __device__ inline size_t blk_ws_doubles(int n) {
     const size_t npad = ((size_t)n + 63) & ~(size_t)63, nt = npad / 64;
-    return 64 + 2 * BLK_NB * npad + npad + nt * (BLK_P1W + 1) + 2 * nt * nt * 64;
+    return 64 + 2 * BLK_NB * npad + npad + nt * (BLK_P1W + 1) + nt * nt * 64 + nt * TRI_NW * 64;
 }
 __host__ __device__ inline int blk_owner(int B, int T) {
     const int p = B % (2 * T);
@@ -855,8 +857,8 @@ __device__ __forceinline__ double blk_x(const double *av, int i, int jj, int n, 
 }
 
 // Partial products of tile (R, C) with x = v_j: row part sum_c A[r][c] x[c]
-// -> Prow[R][C], column part sum_r A[r][c] x[r] -> Pcol[C][R] (off-diagonal
-// tiles only).  One wavefront, lane = column; rows in chunks of 16, which
+// -> Prow[R][C]; the column part sum_r A[r][c] x[r] (lane = column) is
+// returned (zero for a diagonal tile).  One wavefront, lane = column; rows in chunks of 16, which
 // pass through the wave's LDS buffer for the row part (lane = row x quarter).
 // Edge tiles (rows or columns beyond n) load at clamped addresses and zero
 // the values outside the matrix.
@@ -908,9 +910,8 @@ __device__ __forceinline__ void blk_tile_rows(const double *__restrict__ A, int 
     }
 }
 
-__device__ __forceinline__ void blk_tile(const double *__restrict__ A, int n, int nt, int R, int C, int jj, double v0,
-                                         const double *av, double *Prow, double *Pcol, double *Tw, double *xs,
-                                         int lane) {
+__device__ __forceinline__ double blk_tile(const double *__restrict__ A, int n, int nt, int R, int C, int jj,
+                                           double v0, const double *av, double *Prow, double *Tw, double *xs, int lane) {
     const int c = 64 * C + lane;
     const double xc = blk_x(av, c, jj, n, v0);
     const double xr = blk_x(av, 64 * R + lane, jj, n, v0);
@@ -920,12 +921,14 @@ __device__ __forceinline__ void blk_tile(const double *__restrict__ A, int n, in
     double *prow = Prow + ((size_t)R * nt + C) * 64;
     if (64 * C + 63 < n && 64 * R + 63 < n) blk_tile_rows<false>(A, n, R, c, cok, xr, colacc, Tw, xs, lane, prow);
     else blk_tile_rows<true>(A, n, R, c, cok, xr, colacc, Tw, xs, lane, prow);
-    if (R != C) st_dev(Pcol + ((size_t)C * nt + R) * 64 + lane, colacc);
+    return R != C ? colacc : 0.0;
 }
 
 // The workgroup's tiles of the product (tile columns it owns, rows R0..C),
-// flattened over its wavefronts.  Not inlined: the tile loop gets registers
-// of its own.
+// flattened over its wavefronts.  The column parts of block C stay with its
+// owner: each wavefront sums those of its tiles (in row order) and stores the
+// sum to Pcw[C][wavefront] with a plain store (read back by the same
+// wavefront).  Not inlined: the tile loop gets registers of its own.
 template <class P>
 __device__ __forceinline__ P *uni_ptr(P *p) {
     const uint64_t u = (uint64_t)p;
@@ -935,14 +938,15 @@ __device__ __forceinline__ P *uni_ptr(P *p) {
 __device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
 __device__ __noinline__ void blk_symv(const double *A_, int n_, int nt_, int R0_, int rank_, int T_, int jj_, double v0,
-                                      const double *av_, double *Prow_, double *Pcol_) {
+                                      const double *av_, double *Prow_, double *Pcw_) {
     extern __shared__ double lds[];
     // arguments arrive in vector registers; they are uniform, and saying so
     // keeps tile indices and addresses scalar.  So is the wavefront index.
     // The LDS buffers are taken from the LDS array here (a pointer argument
     // would lose its address space and every access would be a flat one).
     const double *A = uni_ptr(A_), *av = uni_ptr(av_);
-    double *Prow = uni_ptr(Prow_), *Pcol = uni_ptr(Pcol_);
+    double *Prow = uni_ptr(Prow_);
+    gdouble *Pcw = (gdouble *)uni_ptr(Pcw_);
     const int n = uni(n_), nt = uni(nt_), R0 = uni(R0_), rank = uni(rank_), T = uni(T_), jj = uni(jj_);
     const int w = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
     double *const Tw = lds + BLK_LDS_TW + w * BLK_TW;
@@ -951,8 +955,10 @@ __device__ __noinline__ void blk_symv(const double *A_, int n_, int nt_, int R0_
     for (int C = R0; C < nt; ++C) {
         if (blk_owner(C, T) != rank) continue;
         const int cnt = C - R0 + 1;
+        double col = 0.0;
         for (int i = ((w - f) % TRI_NW + TRI_NW) % TRI_NW; i < cnt; i += TRI_NW)
-            blk_tile(A, n, nt, R0 + i, C, jj, v0, av, Prow, Pcol, Tw, xs, lane);
+            col += blk_tile(A, n, nt, R0 + i, C, jj, v0, av, Prow, Tw, xs, lane);
+        Pcw[((size_t)C * TRI_NW + w) * 64 + lane] = col;
         f += cnt;
     }
 }
@@ -1051,13 +1057,13 @@ __device__ __noinline__ void tri_blk(const EigMat &mt, int rank, int T, int *bar
     double *const part1 = av + npad;
     double *const part2 = part1 + (size_t)nt * BLK_P1W;
     double *const Prow = part2 + nt;
-    double *const Pcol = Prow + (size_t)nt * nt * 64;
+    double *const Pcw = Prow + (size_t)nt * nt * 64;
     double *const tau_g = vslot(ws, n, V_TAU), *const gdg = vslot(ws, n, V_DG), *const god = vslot(ws, n, V_OD);
     // plain accesses to global memory through global-address-space pointers
     // (global_* instructions rather than flat_*)
     gdouble *const gA = (gdouble *)A;
     const gdouble *const gV = (const gdouble *)Vb, *const gW = (const gdouble *)Wb, *const gav = (const gdouble *)av;
-    const gdouble *const gPcol = (const gdouble *)Pcol;
+    const gdouble *const gPcw = (const gdouble *)Pcw;
     const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     double *const cf = lds + BLK_LDS_CF;
     double *const Kc = cf + 64 * CF_K;
@@ -1250,17 +1256,17 @@ __device__ __noinline__ void tri_blk(const EigMat &mt, int rank, int T, int *bar
             }
         }
         BLK_PROF(5);
-        if (tau != 0.0) blk_symv(A, n, nt, R0, rank, T, j + 1, v0, av, Prow, Pcol);
+        if (tau != 0.0) blk_symv(A, n, nt, R0, rank, T, j + 1, v0, av, Prow, Pcw);
         BLK_PROF(6);
         ++phase;
         if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
         BLK_PROF(7);
-        // ---- P3: p on the owned blocks.  Term i of block B (row part of tile
-        // (B, B+i), then column parts of tiles (R0.., B)) is summed by
-        // wavefront i % 16; the 16 partial sums are added in wavefront order.
+        // ---- P3: p on the owned blocks.  Row part i of block B (tile
+        // (B, B+i)) is summed by wavefront i % 16, which then adds the column
+        // parts its namesake summed in the product; the 16 partial sums are
+        // added in wavefront order.
         {
             double *red = lds + BLK_LDS_TW;  // [block of the group][wavefront][64]
-            const int nterm = nt - R0;
             for (int B0 = R0; B0 < nt;) {
                 // a group of up to 16 owned blocks [B0, B1)
                 int nb = 0, B1 = B0, mine = -1;
@@ -1275,20 +1281,18 @@ __device__ __noinline__ void tri_blk(const EigMat &mt, int rank, int T, int *bar
                     for (int Bb = B0; Bb < B1; ++Bb) {
                         if (blk_owner(Bb, T) != rank) continue;
                         const int nrow = nt - Bb;
-                        double s = 0.0;
-                        for (int i0 = w; i0 < nterm; i0 += 16 * 16) {
+                        double s = gPcw[((size_t)Bb * TRI_NW + w) * 64 + lane];
+                        for (int i0 = w; i0 < nrow; i0 += 16 * 16) {
                             double t[16];
 #pragma unroll
                             for (int u = 0; u < 16; ++u) {
-                                // both candidate loads at valid addresses, then a select
-                                // (no branch, so the 16 items' loads are in flight together)
-                                const int i = i0 + 16 * u, ic = min(i, nterm - 1);
-                                const double pr = ld_dev(Prow + ((size_t)Bb * nt + Bb + min(ic, nrow - 1)) * 64 + lane);
-                                const double pc = gPcol[((size_t)Bb * nt + R0 + max(ic - nrow, 0)) * 64 + lane];
-                                t[u] = i >= nterm ? 0.0 : ic < nrow ? pr : pc;
+                                // clamped addresses, then a select (no branch, so the
+                                // 16 items' loads are in flight together)
+                                const int i = i0 + 16 * u;
+                                t[u] = ld_dev(Prow + ((size_t)Bb * nt + Bb + min(i, nrow - 1)) * 64 + lane);
                             }
 #pragma unroll
-                            for (int u = 0; u < 16; ++u) s += t[u];
+                            for (int u = 0; u < 16; ++u) s += i0 + 16 * u < nrow ? t[u] : 0.0;
                         }
                         red[(b * 16 + w) * 64 + lane] = s;
                         ++b;
