@@ -878,8 +878,13 @@ __device__ __forceinline__ void blk_chunk_load(const gdouble *A, int cc, int rb,
     }
 }
 
+// Row part of a chunk (rows 16q..16q+15 of the tile) through the wave's
+// transpose buffer: the sum of row 16q + r ends up in lanes r, r+16, r+32,
+// r+48, and lane 16q + r keeps it in `rowp`, so after four chunks lane l
+// holds row l of the tile's row partial product (no store per chunk: a
+// store's completion would be waited for with the next chunk's loads).
 __device__ __forceinline__ void blk_chunk_use(const double (&a)[16], int q, double xr, double &colacc, double *Tw,
-                                              const double *xs, int lane, double *prow) {
+                                              const double *xs, int lane, double &rowp) {
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) {
         colacc = fma(a[rr], readlane_d(xr, 16 * q + rr), colacc);
@@ -892,36 +897,44 @@ __device__ __forceinline__ void blk_chunk_use(const double (&a)[16], int q, doub
     for (int cc = 0; cc < 16; ++cc) s = fma(Tw[r2 * BLK_TWS + 16 * g + cc], xs[16 * g + cc], s);
     s += __shfl_xor(s, 16, 64);
     s += __shfl_xor(s, 32, 64);
-    if (lane < 16) st_dev(prow + 16 * q + lane, s);
+    rowp = g == q ? s : rowp;
     wave_lds_sync();
 }
 
+// One tile by one wavefront.  The previous tile's row partial (pend) is
+// stored after this tile's first loads are issued, so no load waits for it.
 template <bool EDGE>
-__device__ __forceinline__ void blk_tile_rows(const double *__restrict__ A, int n, int R, int c, bool cok, double xr,
-                                              double &colacc, double *Tw, const double *xs, int lane, double *prow) {
+__device__ __forceinline__ double blk_tile_t(const double *__restrict__ A, int n, int nt, int R, int C, int jj,
+                                             double v0, const double *av, double *Prow, double *Tw, double *xs,
+                                             int lane, double *&pend_p, double &pend_v) {
     const gdouble *p = (const gdouble *)A;
+    const int c = 64 * C + lane, r0 = 64 * R;
+    const bool cok = c < n;
     const int cc = cok ? c : n - 1;
-    const int r0 = 64 * R;
+    const double xc = blk_x(av, c, jj, n, v0);
+    const double xr = blk_x(av, r0 + lane, jj, n, v0);
+    double a[16];
+    blk_chunk_load<EDGE>(p, cc, r0, n, cok, a);
+    if (pend_p) st_dev(pend_p + lane, pend_v);
+    xs[lane] = xc;
+    double colacc = 0.0, rowp = 0.0;
+    blk_chunk_use(a, 0, xr, colacc, Tw, xs, lane, rowp);
 #pragma unroll 1
-    for (int q = 0; q < 4; ++q) {
-        double a[16];
+    for (int q = 1; q < 4; ++q) {
         blk_chunk_load<EDGE>(p, cc, r0 + 16 * q, n, cok, a);
-        blk_chunk_use(a, q, xr, colacc, Tw, xs, lane, prow);
+        blk_chunk_use(a, q, xr, colacc, Tw, xs, lane, rowp);
     }
+    pend_p = Prow + ((size_t)R * nt + C) * 64;
+    pend_v = rowp;
+    return R != C ? colacc : 0.0;
 }
 
 __device__ __forceinline__ double blk_tile(const double *__restrict__ A, int n, int nt, int R, int C, int jj,
-                                           double v0, const double *av, double *Prow, double *Tw, double *xs, int lane) {
-    const int c = 64 * C + lane;
-    const double xc = blk_x(av, c, jj, n, v0);
-    const double xr = blk_x(av, 64 * R + lane, jj, n, v0);
-    xs[lane] = xc;
-    double colacc = 0.0;
-    const bool cok = c < n;
-    double *prow = Prow + ((size_t)R * nt + C) * 64;
-    if (64 * C + 63 < n && 64 * R + 63 < n) blk_tile_rows<false>(A, n, R, c, cok, xr, colacc, Tw, xs, lane, prow);
-    else blk_tile_rows<true>(A, n, R, c, cok, xr, colacc, Tw, xs, lane, prow);
-    return R != C ? colacc : 0.0;
+                                           double v0, const double *av, double *Prow, double *Tw, double *xs, int lane,
+                                           double *&pend_p, double &pend_v) {
+    if (64 * C + 63 < n && 64 * R + 63 < n)
+        return blk_tile_t<false>(A, n, nt, R, C, jj, v0, av, Prow, Tw, xs, lane, pend_p, pend_v);
+    return blk_tile_t<true>(A, n, nt, R, C, jj, v0, av, Prow, Tw, xs, lane, pend_p, pend_v);
 }
 
 // The workgroup's tiles of the product (tile columns it owns, rows R0..C),
@@ -952,15 +965,17 @@ __device__ __noinline__ void blk_symv(const double *A_, int n_, int nt_, int R0_
     double *const Tw = lds + BLK_LDS_TW + w * BLK_TW;
     double *const xs = lds + BLK_LDS_XS + w * 64;
     int f = 0;
+    double *pend_p = nullptr, pend_v = 0.0;
     for (int C = R0; C < nt; ++C) {
         if (blk_owner(C, T) != rank) continue;
         const int cnt = C - R0 + 1;
         double col = 0.0;
         for (int i = ((w - f) % TRI_NW + TRI_NW) % TRI_NW; i < cnt; i += TRI_NW)
-            col += blk_tile(A, n, nt, R0 + i, C, jj, v0, av, Prow, Tw, xs, lane);
+            col += blk_tile(A, n, nt, R0 + i, C, jj, v0, av, Prow, Tw, xs, lane, pend_p, pend_v);
         Pcw[((size_t)C * TRI_NW + w) * 64 + lane] = col;
         f += cnt;
     }
+    if (pend_p) st_dev(pend_p + lane, pend_v);
 }
 
 // SYR2K of the finished panel on tile (R, C): A[r][c] -= sum_k V[r][k] W[c][k]
