@@ -2696,7 +2696,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         HIPCHK(c, hipFuncSetAttribute((const void *)k_eig_tri_coop, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)TRI_COOP_LDS));
         if (const char *e = getenv("NRG_BLK_MIN_N")) c->blk_min_n = std::max(3, atoi(e));
-        if (const char *e = getenv("NRG_BLK_TDIV")) c->blk_tdiv = std::max(1, atoi(e));
+        if (const char *e = getenv("NRG_BLK_TDIV")) c->blk_tdiv = std::max(0, atoi(e));
         HIPCHK(c, hipFuncSetAttribute((const void *)k_eig_tri_blk, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)TRI_COOP_LDS));
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_coop, TRI_NT, TRI_COOP_LDS));
@@ -2737,19 +2737,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             continue;
         }
         if (tasks[i].n >= blk_min_n && blk_ws_doubles(tasks[i].n) <= (size_t)tasks[i].n * tasks[i].n) {
-            // blocked reduction: about two 64-column blocks per rank; teams
-            // above one XCD's share span XCDs.  Per step three barriers, then
-            // the rank's share of the half matrix (4 (n-j)^2 bytes at ~25 GB/s
-            // per CU).
-            const int nt = (tasks[i].n + 63) / 64;
-            int T = std::max(2, nt / blk_tdiv);
-            if (T > std::min(TEAM_MAX, per_xcd)) {
-                T = std::min({T, TEAM_GLOBAL_MAX, tri_grid});
-                global[i] = 1;
-            }
-            teamT[i] = T;
-            blkjob[i] = 1;
-            cost[i] = n * (global[i] ? 12.0 : 4.5) + 5.3e-5 * n * n * n / T;
+            blkjob[i] = 1;  // team size below, once all blocked jobs are known
             continue;
         }
         const int nsl = (tasks[i].n - 1 + TEAM_W - 1) / TEAM_W;
@@ -2763,6 +2751,62 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         teamT[i] = std::max(T, 2);
         cost[i] = n * (3.0 + 1.1e-4 * n * n / teamT[i]);
         mats[i].pad = 1;  // reflectors to copy from S2
+    }
+    // Blocked reductions: a team of T takes about o(n) + W(n) / T
+    // microseconds: o(n) = n (0.0114 n - 25) of per-column latency that no
+    // team size removes (barriers, block sums), and W(n) = 0.266 n^2 +
+    // 7e-5 n^3 CU-us of work split over the ranks (17 us per column and
+    // owned 64-column block for the column and p, and the half-matrix
+    // product).  Fitted to per-phase clock probes on MI355X with the
+    // keep-10000 blocks running together.  The team sizes make every blocked
+    // job finish by a common time M, the smallest for which all teams fit on
+    // the device together (bisection on M), so CUs left over by the smallest
+    // sizes go to the largest matrices (a single 13136-row block: 51 -> 103
+    // ranks, 4.24 -> 2.73 s); when even the smallest sizes do not fit at
+    // once (the keep-10000 iterations) every team has its smallest size and
+    // the queues run them in turn.  NRG_BLK_TDIV > 0 instead gives every
+    // matrix nt / NRG_BLK_TDIV ranks.
+    {
+        auto blk_o = [](double n) { return n * std::max(0.0114 * n - 25.0, 10.0); };
+        auto blk_w = [](double n) { return 0.266 * n * n + 7e-5 * n * n * n; };
+        const int tmax = std::min(TEAM_GLOBAL_MAX, tri_grid);
+        // between one rank per four and one per two 64-column blocks: smaller
+        // teams measured slower than the model in a full mix (their ranks'
+        // per-block work grows), larger ones were not measured
+        auto team_for = [&](double n, double M) {
+            const int nt = ((int)n + 63) / 64;
+            const int cap = std::max(2, std::min(tmax, nt / 2)), floor_ = std::min(cap, std::max(2, nt / 4));
+            const double room = M - blk_o(n);
+            if (room <= 0.0) return cap;
+            return (int)std::min<double>(cap, std::max<double>(floor_, std::ceil(blk_w(n) / room)));
+        };
+        std::vector<int> bj;
+        for (int i = 0; i < ntasks; ++i)
+            if (blkjob[i]) bj.push_back(i);
+        double M = 0.0;
+        if (!bj.empty() && blk_tdiv <= 0) {
+            double lo = 0.0, hi = 0.0;
+            for (int i : bj) {
+                lo = std::max(lo, blk_o(tasks[i].n));
+                hi = std::max(hi, blk_o(tasks[i].n) + blk_w(tasks[i].n) / 2.0);
+            }
+            for (int it = 0; it < 60; ++it) {
+                const double mid = 0.5 * (lo + hi);
+                long sum = 0;
+                for (int i : bj) sum += team_for(tasks[i].n, mid);
+                (sum <= tri_grid ? hi : lo) = mid;
+            }
+            M = hi;
+        }
+        for (int i : bj) {
+            const double n = tasks[i].n;
+            const int nt = (tasks[i].n + 63) / 64;
+            int T = blk_tdiv > 0 ? std::max(2, nt / blk_tdiv) : team_for(n, M);
+            T = std::min(T, tmax);
+            if (T > std::min(TEAM_MAX, per_xcd)) global[i] = 1;
+            teamT[i] = T;
+            cost[i] = blk_o(n) + blk_w(n) / T;
+        }
     }
     std::vector<int> order(ntasks);
     for (int i = 0; i < ntasks; ++i) order[i] = i;

@@ -43,8 +43,9 @@ struct nrg_ctx {
     int reg_capacity = 0;
     // matrices from this dimension up use the blocked tridiagonalisation
     int blk_min_n = 1024;
-    // blocked teams: one workgroup per blk_tdiv 64-column blocks
-    int blk_tdiv = 4;
+    // blocked teams: sized by the makespan model (0), or one workgroup per
+    // blk_tdiv 64-column blocks
+    int blk_tdiv = 0;
     // grouped GEMM: partial tiles of split outputs
     double *gpart = nullptr;
     size_t gpart_tiles = 0;
