@@ -991,6 +991,18 @@ __device__ __forceinline__ void blk_syr2k_tile(double *__restrict__ A, int n, in
     double *const Ls = lds + BLK_LDS_TW, *const Rs = Ls + 2 * BLK_NB * 64;
     const double *const Kc = lds + BLK_LDS_CF + 64 * CF_K;
     const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int bi = w >> 2, bj = w & 3;
+    const int col = 64 * C + 16 * bj + (lane & 15);
+    const int row0 = 64 * R + 16 * bi + (lane >> 4);  // element v: row row0 + 4 v
+    // the old tile first: its loads are in flight while the operands are staged
+    blk_d4 acc;
+    gdouble *Ag = (gdouble *)A;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const double x = Ag[(size_t)min(row0 + 4 * v, n - 1) * n + min(col, n - 1)];
+        acc[v] = (row0 + 4 * v < n && col < n) ? x : 0.0;
+    }
 #pragma unroll
     for (int q = 0; q < 2 * BLK_NB * 64 / TRI_NT; ++q) {
         const int e = tid + q * TRI_NT, k = e >> 6, i = e & 63;
@@ -1006,17 +1018,6 @@ __device__ __forceinline__ void blk_syr2k_tile(double *__restrict__ A, int n, in
         if (loadR) Rs[e] = k < BLK_NB ? pc - Kc[kk] * vc : vc;
     }
     __syncthreads();
-    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-    const int bi = w >> 2, bj = w & 3;
-    const int col = 64 * C + 16 * bj + (lane & 15);
-    const int row0 = 64 * R + 16 * bi + (lane >> 4);  // element v: row row0 + 4 v
-    blk_d4 acc;
-    gdouble *Ag = (gdouble *)A;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const double x = Ag[(size_t)min(row0 + 4 * v, n - 1) * n + min(col, n - 1)];
-        acc[v] = (row0 + 4 * v < n && col < n) ? x : 0.0;
-    }
 #pragma unroll
     for (int s = 0; s < 2 * BLK_NB / 4; ++s) {
         const int k = 4 * s + (lane >> 4);
