@@ -1489,6 +1489,7 @@ struct Engine {
                 std::vector<TraceCheck> checks;
                 RhoMap cur = rho_init;
                 for (int N = Nmax - 1; N > P.Ninit; --N) {
+                    if (!opt.quiet) std::cout << "DM sweep N=" << N << std::endl;
                     use(N);
                     Arena tmp(&gpu);
                     RhoMap prev = dm_step(N, cur, nullptr, *rho_arena, tmp);
@@ -1504,6 +1505,7 @@ struct Engine {
                 std::vector<TraceCheck> checks;
                 RhoMap cur = init_rho_FDM(Nmax - 1);
                 for (int N = Nmax - 1; N > P.Ninit; --N) {
+                    if (!opt.quiet) std::cout << "FDM sweep N=" << N << std::endl;
                     use(N);
                     Arena tmp(&gpu);
                     const bool last = (N + 1 == Nmax);
@@ -1696,6 +1698,7 @@ struct Engine {
         if (P.calc0) dump_energies(fen2, P.Ninit - 1, std::map<Invar, std::vector<double>>(D.energies.begin(), D.energies.end()));
         for (int N = P.Ninit; N < Nmax; ++N) {
             const bool last = (N + 1 == Nmax);
+            if (!opt.quiet) std::cout << "Spectra N=" << N << std::endl;
             use(N);
             Iter &it = iters[N];
             if (fen2.is_open()) {
