@@ -808,8 +808,10 @@ __global__ __launch_bounds__(REG_NT) void k_eig_tri_reg(const EigMat *mats, cons
 //           bits everywhere); owners: partial products of the tiles -> barrier
 //       P3  owners: p = tau (A v - V W^T v - W V^T v) from the partial
 //           products in a fixed order, per-block p.v                -> barrier
-//     Every sum runs per 64-block in an order fixed by (n, j) alone, so the
-//     result does not depend on the team size.
+//     Every sum runs in an order fixed by (n, j) and the team size, so a
+//     matrix reduced by a team of the same size gives the same bits whatever
+//     else runs beside it; another team size changes the column parts' order
+//     of summation (per wavefront of the owner) and so the last bits.
 //     Output as tri_team: reflector j in row j of A (columns j+1.., first
 //     element included), tau, diagonal and off-diagonal of T.
 // ---------------------------------------------------------------------------

@@ -46,9 +46,10 @@ static int nrg_main_args(int argc, char **argv) {
         else if (a == "--plain-diag") opt.plain = true;
         else if (a == "--hbm-budget") opt.hbm_budget_gb = std::atof(next().c_str());
         else if (a == "--spill-all") opt.hbm_budget_gb = 0.0;
+        else if (a == "--spill-pageable") opt.spill_pageable = true;
         else if (a == "--iter-log") opt.iter_log = next();
         else if (a == "-h" || a == "--help") {
-            std::cout << "usage: nrg [-w workdir] [--dump-eigen file] [--stats file [--stats-light]] [--tables dir] [--device n] [--quiet] [--plain-diag] [--hbm-budget GB] [--spill-all] [--iter-log file]\n";
+            std::cout << "usage: nrg [-w workdir] [--dump-eigen file] [--stats file [--stats-light]] [--tables dir] [--device n] [--quiet] [--plain-diag] [--hbm-budget GB] [--spill-all] [--spill-pageable] [--iter-log file]\n";
             return 0;
         } else nrg::die("unknown option " + a);
     }

@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -292,8 +293,7 @@ struct Engine {
     // pool, and spilled eigenvectors release their pinned host memory
     ~Engine() {
         (void)nrg_sync(gpu.ctx);
-        for (auto &it : iters)
-            if (it.host) (void)nrg_host_free(gpu.ctx, it.host);
+        for (auto &it : iters) free_host(it);
     }
 
     double SCALE(int N) const {
@@ -822,9 +822,18 @@ struct Engine {
             if (schedule[k] == N) return k;
         return SIZE_MAX;
     }
+    // The host copy is pinned while pinned memory lasts (the runtime refuses
+    // it beyond a limit of its own, measured at about 66 GB on the MI355X
+    // box), then ordinary pageable memory, which the runtime copies through
+    // staging buffers.
     void spill(Iter &it) {
         it.host = nullptr;
-        gpu.check(nrg_host_alloc(gpu.ctx, it.bytes, &it.host), "nrg_host_alloc");
+        it.host_pinned = !pinned_exhausted && nrg_host_alloc(gpu.ctx, it.bytes, &it.host) == NRG_OK;
+        if (!it.host_pinned) {
+            pinned_exhausted = true;
+            it.host = std::malloc(it.bytes);
+            if (!it.host) die("spill: out of host memory for " + std::to_string(it.bytes) + " bytes of eigenvectors");
+        }
         gpu.check(nrg_d2h(gpu.ctx, it.host, it.base, it.bytes), "nrg_d2h");
         it.arena.reset();
         gpu.trim();
@@ -835,8 +844,7 @@ struct Engine {
     void restore(Iter &it) {
         double *nb = gpu.alloc(it.bytes / 8);
         gpu.check(nrg_h2d(gpu.ctx, nb, it.host, it.bytes), "nrg_h2d");
-        gpu.check(nrg_host_free(gpu.ctx, it.host), "nrg_host_free");
-        it.host = nullptr;
+        free_host(it);
         for (auto &kv : it.subs) {
             kv.second.U = nb + (kv.second.U - it.base);
             kv.second.W = nb + (kv.second.W - it.base);
@@ -848,6 +856,13 @@ struct Engine {
         resident += it.bytes;
         ++restores;
     }
+    void free_host(Iter &it) {
+        if (!it.host) return;
+        if (it.host_pinned) (void)nrg_host_free(gpu.ctx, it.host);
+        else std::free(it.host);
+        it.host = nullptr;
+    }
+    bool pinned_exhausted = opt.spill_pageable;  // a pinned allocation failed: pageable from then on
     // Spill resident iterations (the one used last first) until `need` more
     // bytes fit in the budget; iteration `keep` and, in pass 1, its
     // predecessor (E_prev) stay.
@@ -1903,8 +1918,7 @@ struct Engine {
             write_spectra();
             if (P.fdmexpv) write_customfdm();
         }
-        for (auto &it : iters)
-            if (it.host) gpu.check(nrg_host_free(gpu.ctx, it.host), "nrg_host_free");
+        for (auto &it : iters) free_host(it);
         iters.clear();
         rhos.clear();
         rhofdm.clear();

@@ -180,7 +180,8 @@ struct Iter {
     // not fit in the device budget (the reference's workdir/unitaryN files)
     double *base = nullptr;  // device chunk while resident
     size_t bytes = 0;
-    void *host = nullptr;    // pinned copy while spilled
+    void *host = nullptr;    // host copy while spilled
+    bool host_pinned = true; // pinned (else pageable) host copy
     bool spilled = false;
 };
 
@@ -205,6 +206,7 @@ struct Options {
     bool quiet = false;  // no per-iteration log lines
     bool plain = false;  // one eigensolver call per block (the unbatched GPU path), for comparison
     double hbm_budget_gb = -1.0;  // device bytes for resident eigenvectors; <0: 60% of free HBM at start
+    bool spill_pageable = false;  // spill to pageable host memory from the start (a test of that path)
     std::string iter_log;         // per first-pass iteration: N, seconds since start (stream drained), eigenpairs
     DiagService *diag = nullptr;  // shared eigensolver (serving loop); null: own batched calls
 };

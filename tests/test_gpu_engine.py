@@ -47,17 +47,19 @@ def test_engine_siam_fdm_keep300(tmp_path):
     _run(str(tmp_path), c)
 
 
-def test_engine_spill_to_host_matches_oracle(tmp_path):
-    """Eigenvectors spilled to pinned host memory after every iteration and
-    brought back for the DM/FDM sweeps and pass 2 give the oracle's results."""
+@pytest.mark.parametrize("pageable", [False, True])
+def test_engine_spill_to_host_matches_oracle(tmp_path, pageable):
+    """Eigenvectors spilled to host memory after every iteration (pinned, or
+    pageable as once pinned memory runs out) and brought back for the DM/FDM
+    sweeps and pass 2 give the oracle's results."""
     import json
     c = dict(sym="QS", nch=1, Nmax=20, model_kw=dict(U=1.0, eps_d=-0.5, Gamma=0.04),
              param=dict(keep=300, keepenergy=8.0, T=1e-6, fdm=True, dmnrg=True, specd="A_d-A_d", bins=50,
                         savebins=True))
     wd = str(tmp_path)
     nrgio.make_case(wd, sym=c["sym"], nch=c["nch"], Nmax=c["Nmax"], model_kw=c["model_kw"], **c["param"])
-    print("case spill-all", wd, flush=True)
-    nrgio.run_engine(NRG, wd, extra=("--spill-all",))
+    print("case spill-all pageable=%s" % pageable, wd, flush=True)
+    nrgio.run_engine(NRG, wd, extra=("--spill-all",) + (("--spill-pageable",) if pageable else ()))
     st = json.load(open(os.path.join(wd, "stats.json")))
     assert st["spills"] > 0 and st["restores"] > 0
     o = load_oracle().Oracle(wd).run()

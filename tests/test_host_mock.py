@@ -82,12 +82,14 @@ def test_missing_table_entry_is_an_error(tmp_path):
 @pytest.mark.parametrize("case", ["qs_dmnrg", "qs_fdm"])
 def test_spill_to_host_changes_nothing(tmp_path, case):
     """With a zero device budget every iteration's eigenvectors go to host
-    memory after use and come back for the sweeps and pass 2: all outputs are
-    identical to the resident run."""
+    memory after use (pinned, or pageable as after pinned memory runs out)
+    and come back for the sweeps and pass 2: all outputs are identical to the
+    resident run."""
     import json
     c = CASES[case]
     outs = []
-    for tag, extra in (("resident", ()), ("spilled", ("--spill-all",))):
+    for tag, extra in (("resident", ()), ("spilled", ("--spill-all",)),
+                       ("pageable", ("--spill-all", "--spill-pageable"))):
         wd = str(tmp_path / tag)
         os.makedirs(wd)
         nrgio.make_case(wd, sym=c["sym"], nch=c["nch"], Nmax=c["Nmax"], **c["param"])
@@ -95,11 +97,11 @@ def test_spill_to_host_changes_nothing(tmp_path, case):
         st = json.load(open(os.path.join(wd, "stats.json")))
         files = sorted(f for f in os.listdir(wd) if f.startswith(("td", "custom", "spec_")))
         outs.append((st, {f: open(os.path.join(wd, f), "rb").read() for f in files}))
-    (st0, f0), (st1, f1) = outs
-    assert st0["spills"] == 0 and st1["spills"] > 0 and st1["restores"] > 0
-    assert f0.keys() == f1.keys() and len(f0) >= 3
+    (st0, f0), (st1, f1), (st2, f2) = outs
+    assert st0["spills"] == 0 and st1["spills"] > 0 and st1["restores"] > 0 and st2["spills"] == st1["spills"]
+    assert f0.keys() == f1.keys() == f2.keys() and len(f0) >= 3
     for f in f0:
-        assert f0[f] == f1[f], f
+        assert f0[f] == f1[f] == f2[f], f
 
 
 def _run_rc(wd, extra=()):
