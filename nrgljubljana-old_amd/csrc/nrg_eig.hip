@@ -89,7 +89,7 @@ struct EigMat {
     double *W;
     double *ws;
     int n;
-    int pad;  // 1: reduced by a team (reflectors in S2)
+    int pad;  // 1: reduced by a team (reflectors in S2); 2: two-stage reduction
 };
 
 struct DcMerge {
@@ -180,7 +180,7 @@ constexpr int BAR_STRIDE = 32;                    // ints between team barrier c
 
 struct TriJob {
     int mat, T;
-    int blocked;  // 1: blocked reduction (tri_blk), 0: streaming rows (tri_team)
+    int blocked;  // 2: two-stage (sbr_blk), 1: blocked reduction (tri_blk), 0: streaming rows (tri_team)
     int pad;
 };
 
@@ -1361,6 +1361,494 @@ __device__ __noinline__ void tri_blk(const EigMat &mt, int rank, int T, int *bar
     }
 }
 
+// ---------------------------------------------------------------------------
+// 1e. two-stage reduction, stage 1 (sbr_blk): dense -> symmetric band of
+//     half-width SBR_B (successive band reduction; the band then goes to
+//     tridiagonal form by bulge chasing, section 1f).  The whole symmetric
+//     matrix is kept (the upper triangle is mirrored into the lower one at the
+//     start), so a block of 64 rows holds every element of its rows; team
+//     rank blk_owner(B) alone touches the rows of block B.  Panel p (columns
+//     j0 = SBR_B p .. j0 + SBR_B - 1, trailing range [c0 = j0 + SBR_B, n)):
+//       QR of the panel's lower copy A[c0:n, j0:j0+SBR_B] with its rows
+//         spread over their owners (registers, lane = row): per reflector one
+//         barrier for the partial sums |x|^2 and x . a_i of every block,
+//         summed in block order by every rank (the same bits everywhere);
+//       Y = A22 V on the owned tile rows (FP64 MFMA), per-block Gram
+//         partials V^T V and V^T Y -> barrier -> every rank forms T (dlarft)
+//         and S = T^T (V^T Y) T and writes W = Y T - V S / 2 for its rows
+//         -> barrier;
+//       A22 -= V W^T + W V^T on the owned tile rows (both triangles, MFMA).
+//     Output: row j0 + k of S2 holds R at columns c0 .. c0 + k (beta at
+//     c0 + k) and reflector k at columns c0 + k + 1 .. n - 1 (implicit 1 at
+//     c0 + k); tau in V_TAU; A's upper triangle the rest of the band
+//     (columns j .. c0 - 1 of a panel's row j, all of the rows below the
+//     last panel).
+// ---------------------------------------------------------------------------
+constexpr int SBR_B = 32;
+constexpr int SBR_MAXQ = 8;  // owned blocks of a trailing range (wavefronts holding panel rows)
+// S1 layout (doubles from a 512-byte aligned start): V, W (SBR_B x npad,
+// reflector-major), Y (npad x SBR_B), QR partial sums (2 x nt x SBR_B),
+// pivot rows (2 x SBR_B), Gram partials (nt x 2 x SBR_B^2) and their totals
+__host__ __device__ inline size_t sbr_ws_doubles(int n) {
+    const size_t npad = ((size_t)n + 63) & ~(size_t)63, nt = npad / 64;
+    return 64 + 3 * (size_t)SBR_B * npad + 2 * nt * SBR_B + 2 * SBR_B + (nt + 1) * 2 * SBR_B * SBR_B;
+}
+// reflectors of stage 1: SBR_B per panel, panels while the trailing range
+// [SBR_B (p + 1), n) has at least two rows
+__host__ __device__ inline int sbr_nrefl(int n) { return n >= SBR_B + 2 ? SBR_B * ((n - SBR_B - 2) / SBR_B + 1) : 0; }
+// ranks of a stage-1 team: at most SBR_MAXQ owned blocks per rank
+__host__ __device__ inline int sbr_owned_max(int nt, int T) {
+    int mx = 0;
+    for (int r = 0; r < T; ++r) {
+        int c = 0;
+        for (int B = 0; B < nt; ++B) c += blk_owner(B, T) == r;
+        mx = c > mx ? c : mx;
+    }
+    return mx;
+}
+// LDS map (doubles)
+constexpr int SB_LA = 0;                      // tile (64 x 65) of the product
+constexpr int SB_LV = SB_LA + 64 * 65;        // V of the tile's columns (32 x 65)
+constexpr int SB_RED = SB_LV + 32 * 65;       // halves of the product (8 x 256)
+constexpr int SB_T = SB_RED + 8 * 256;        // T, S, G, M (32 x 33 each)
+constexpr int SB_S = SB_T + 32 * 33;
+constexpr int SB_G = SB_S + 32 * 33;
+constexpr int SB_M = SB_G + 32 * 33;
+constexpr int SB_CO = SB_M + 32 * 33;         // per-reflector coefficients, taus
+constexpr int SB_PS = SB_CO + 128;            // partial-sum chunks (32 x 33)
+constexpr int SB_TOT = SB_PS + 32 * 33;          // their totals
+constexpr int SB_END = SB_TOT + 32;
+constexpr int SB_LS = 0, SB_RS = 64 * 64;     // rank-2k operands (64 x 64 each, k-major)
+static_assert(SB_RS + 64 * 64 <= SB_T, "rank-2k operands overlap only the product's buffers");
+static_assert(sizeof(double) * SB_END <= TRI_COOP_LDS, "stage 1 fits the team launch's LDS");
+
+// A[r][c] -= sum_k V[r][k] W[c][k] + W[r][k] V[c][k] on tile (C rows, R
+// columns), rows and columns >= c0.  Ls (row operands -[V W] of block C) is
+// staged by the caller; Rs ([W V] at columns R) here.  Wavefront w computes
+// the 16 x 16 block (w / 4, w % 4), the old tile as the accumulator.
+__device__ __forceinline__ void sbr_syr2k_tile(double *__restrict__ A, int n, int npad, int C, int R, int c0,
+                                               const double *Vb, const double *Wb) {
+    extern __shared__ double lds[];
+    double *const Ls = lds + SB_LS, *const Rs = lds + SB_RS;
+    const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int bi = w >> 2, bj = w & 3;
+    const int col = 64 * R + 16 * bj + (lane & 15);
+    const int row0 = 64 * C + 16 * bi + (lane >> 4);  // element v: row row0 + 4 v
+    blk_d4 acc;
+    gdouble *Ag = (gdouble *)A;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const double x = Ag[(size_t)min(row0 + 4 * v, n - 1) * n + min(col, n - 1)];
+        acc[v] = (row0 + 4 * v < n && col < n) ? x : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 2 * SBR_B * 64 / TRI_NT; ++q) {
+        const int e = tid + q * TRI_NT, k = e >> 6, i = e & 63;
+        const int kk = k < SBR_B ? k : k - SBR_B;
+        const int c = 64 * R + i;
+        Rs[e] = k < SBR_B ? ld_dev(Wb + (size_t)kk * npad + c) : ld_dev(Vb + (size_t)kk * npad + c);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 2 * SBR_B / 4; ++s) {
+        const int k = 4 * s + (lane >> 4);
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ls[k * 64 + 16 * bi + (lane & 15)], Rs[k * 64 + 16 * bj + (lane & 15)],
+                                                   acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int r = row0 + 4 * v;
+        if (r >= c0 && r < n && col >= c0 && col < n) Ag[(size_t)r * n + col] = acc[v];
+    }
+    __syncthreads();
+}
+
+// the number of owned blocks in [B0, nt), and the q-th of them (-1: none)
+__device__ __forceinline__ int sbr_nowned(int B0, int nt, int rank, int T) {
+    int nq = 0;
+    for (int B = B0; B < nt; ++B) nq += blk_owner(B, T) == rank;
+    return nq;
+}
+__device__ __forceinline__ int sbr_qblock(int B0, int nt, int rank, int T, int q) {
+    for (int B = B0; B < nt; ++B)
+        if (blk_owner(B, T) == rank && q-- == 0) return B;
+    return -1;
+}
+
+__device__ __noinline__ void sbr_blk(const EigMat &mt, int rank, int T, int *bar, int *status, int *s_flag) {
+    extern __shared__ double lds[];
+    const int n = mt.n, npad = (n + 63) & ~63, nt = npad >> 6;
+    double *__restrict__ A = mt.A;
+    double *ws = mt.ws;
+    double *const base = (double *)(((uintptr_t)(ws + (size_t)n * n) + 511) & ~(uintptr_t)511);
+    double *const Vb = base;
+    double *const Wb = Vb + (size_t)SBR_B * npad;
+    double *const Yb = Wb + (size_t)SBR_B * npad;
+    double *const part = Yb + (size_t)SBR_B * npad;  // [parity][block][SBR_B]
+    double *const piv = part + 2 * (size_t)nt * SBR_B;  // [parity][SBR_B]
+    double *const gram = piv + 2 * SBR_B;             // [block][2][SBR_B][SBR_B]
+    double *const gtot = gram + (size_t)nt * 2 * SBR_B * SBR_B;  // [2][SBR_B][SBR_B]
+    double *const tau_g = vslot(ws, n, V_TAU);
+    gdouble *const gA = (gdouble *)A;
+    const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    double *const co = lds + SB_CO;  // [0..31] s_i, [32] tau, [33] beta, [34] scale, [64..95] taus
+    int phase = 0;
+#ifdef NRG_SBR_PROF
+    long long pacc[12] = {0}, plast = wall_clock64();
+#define SB_PROF(i) do { __syncthreads(); if (tid == 0) { const long long now_ = wall_clock64(); pacc[i] += now_ - plast; plast = now_; } } while (0)
+#else
+#define SB_PROF(i) do {} while (0)
+#endif
+
+    // 0. mirror the upper triangle into the lower one on the owned rows:
+    //    tile (C', B) (another rank's rows, written before this launch) is
+    //    read whole and written transposed into tile (B, C')
+    for (int B = 0; B < nt; ++B) {
+        if (blk_owner(B, T) != rank) continue;
+        for (int Cp = 0; Cp <= B; ++Cp) {
+            double *const La = lds + SB_LA;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = tid + TRI_NT * u, rr = e >> 6, cc = e & 63;  // row 64 Cp + rr, column 64 B + cc
+                const int r = 64 * Cp + rr, c = 64 * B + cc;
+                La[rr * 65 + cc] = (r < n && c < n) ? gA[(size_t)r * n + c] : 0.0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = tid + TRI_NT * u, ii = e >> 6, cc = e & 63;  // A[64 B + ii][64 Cp + cc]
+                const int r = 64 * B + ii, c = 64 * Cp + cc;
+                if (r < n && c < n && c < r) gA[(size_t)r * n + c] = La[cc * 65 + ii];
+            }
+            __syncthreads();
+        }
+    }
+
+    SB_PROF(0);
+    for (int j0 = 0;; j0 += SBR_B) {
+        const int c0 = j0 + SBR_B, m = n - c0;
+        if (m < 2) break;
+        const int Bt = c0 >> 6;
+        const int kmax = min(SBR_B, m - 1);
+        const int nq = min(sbr_nowned(Bt, nt, rank, T), SBR_MAXQ);
+        // ---- QR of the panel: wavefront q < nq holds rows of the q-th owned
+        // block (lane = row), the panel's 32 columns in registers
+        const bool qw = w < nq;
+        const int myB = qw ? sbr_qblock(Bt, nt, rank, T, w) : 0;
+        const int r = 64 * myB + lane;
+        const bool rok = qw && r >= c0 && r < n;
+        double p[SBR_B];
+#pragma unroll
+        for (int i = 0; i < SBR_B; ++i) p[i] = rok ? gA[(size_t)r * n + j0 + i] : 0.0;
+        for (int k = 0; k < kmax; ++k) {
+            const int par = k & 1, rp = c0 + k;
+            double xk = 0.0;
+#pragma unroll
+            for (int i = 0; i < SBR_B; ++i) xk = i == k ? p[i] : xk;
+            if (qw) {
+                const bool below = rok && r > rp;
+                const double sg = wave_sum(below ? xk * xk : 0.0);
+                if (lane == 0) st_dev(part + ((size_t)par * nt + myB) * SBR_B, sg);
+#pragma unroll
+                for (int i = 1; i < SBR_B; ++i) {
+                    if (i <= k) continue;
+                    const double d = wave_sum(below ? xk * p[i] : 0.0);
+                    if (lane == 0) st_dev(part + ((size_t)par * nt + myB) * SBR_B + i, d);
+                }
+                if (r == rp) {
+#pragma unroll
+                    for (int i = 0; i < SBR_B; ++i)
+                        if (i >= k) st_dev(piv + par * SBR_B + i, p[i]);
+                }
+            }
+            SB_PROF(1);
+            ++phase;
+            if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
+            SB_PROF(2);
+            // block sums in block order: value i (0: |x|^2) by lanes i, blocks
+            // B = Bs + g + 32 u by thread group g (loads issued together), the
+            // 32 group sums added in order
+            {
+                const int Bs = (rp + 1) >> 6;
+                const int i = tid & 31, g = tid >> 5;
+                double s = 0.0;
+                if (i == 0 || i > k) {
+                    double t[16];
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) {
+                        const int B = min(Bs + g + 32 * u, nt - 1);
+                        t[u] = ld_dev(part + ((size_t)par * nt + B) * SBR_B + i);
+                    }
+#pragma unroll
+                    for (int u = 0; u < 16; ++u) s += Bs + g + 32 * u < nt ? t[u] : 0.0;
+                }
+                lds[SB_PS + g * 33 + i] = s;
+                __syncthreads();
+                if (tid < 32) {
+                    double t = 0.0;
+                    for (int gg = 0; gg < 32; ++gg) t += lds[SB_PS + gg * 33 + tid];
+                    lds[SB_TOT + tid] = t;
+                }
+                __syncthreads();
+            }
+            if (tid < 32) {
+                const double sigma = lds[SB_TOT];
+                const double alpha = ld_dev(piv + par * SBR_B + k);
+                double tau = 0.0, beta = alpha, scale = 0.0;
+                if (sigma != 0.0) {
+                    beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+                    tau = (beta - alpha) / beta;
+                    scale = 1.0 / (alpha - beta);
+                }
+                const int i = tid;
+                double si = 0.0;
+                if (i > k) si = tau * (ld_dev(piv + par * SBR_B + i) + lds[SB_TOT + i] * scale);
+                co[i] = si;
+                if (i == 0) {
+                    co[32] = tau;
+                    co[33] = beta;
+                    co[34] = scale;
+                    co[64 + k] = tau;
+                    if (rank == 0) tau_g[j0 + k] = tau;
+                }
+            }
+            __syncthreads();
+            if (qw) {
+                const double scale = co[34], beta = co[33];
+                const double v = r > rp ? xk * scale : 1.0;
+#pragma unroll
+                for (int i = 0; i < SBR_B; ++i) {
+                    if (i == k) p[i] = r > rp ? v : r == rp ? beta : p[i];
+                    else if (i > k && (r >= rp) && rok) p[i] -= co[i] * v;
+                }
+            }
+            __syncthreads();  // co is rewritten by the next reflector
+            SB_PROF(3);
+        }
+        for (int k = kmax + tid; k < SBR_B; k += TRI_NT) {
+            co[64 + k] = 0.0;
+            if (rank == 0) tau_g[j0 + k] = 0.0;
+        }
+        // write-back: the panel rows' R and reflectors into S2 (the rows of
+        // A belong to another rank, whose XCD's L2 may hold dirty lines of
+        // them; k_sbr_band copies them into A after this launch), and V
+        if (qw) {
+            double *const P2 = ws + 2 * (size_t)n * n;
+#pragma unroll
+            for (int i = 0; i < SBR_B; ++i) {
+                if (rok) P2[(size_t)(j0 + i) * n + r] = p[i];
+                const double v = (i < kmax && rok) ? (r > c0 + i ? p[i] : r == c0 + i ? 1.0 : 0.0) : 0.0;
+                st_dev(Vb + (size_t)i * npad + r, v);
+            }
+        }
+        ++phase;
+        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
+        SB_PROF(4);
+        // ---- Y = A22 V on the owned tile rows: wavefront w computes the 16 x 16
+        // block (w & 3, (w >> 2) & 1) of Y[C] over half h = w >> 3 of each
+        // tile's 64 columns; the halves are added in order at the end
+        for (int q = 0; q < nq; ++q) {
+            const int C = sbr_qblock(Bt, nt, rank, T, q);
+            const int bi = w & 3, bj = (w >> 2) & 1, h = w >> 3;
+            blk_d4 acc = {0.0, 0.0, 0.0, 0.0};
+            double ta[4], tv[2];
+            auto fetch = [&](int R) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int e = tid + TRI_NT * u, rr = e >> 6, cc = e & 63;
+                    const int rw = 64 * C + rr, cl = 64 * R + cc;
+                    const double x = gA[(size_t)min(rw, n - 1) * n + min(cl, n - 1)];
+                    ta[u] = (rw < n && cl < n) ? x : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int e = tid + TRI_NT * u, kk = e >> 6, cc = e & 63;
+                    tv[u] = ld_dev(Vb + (size_t)kk * npad + 64 * R + cc);
+                }
+            };
+            fetch(Bt);
+            for (int R = Bt; R < nt; ++R) {
+                __syncthreads();
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int e = tid + TRI_NT * u;
+                    lds[SB_LA + (e >> 6) * 65 + (e & 63)] = ta[u];
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int e = tid + TRI_NT * u;
+                    lds[SB_LV + (e >> 6) * 65 + (e & 63)] = tv[u];
+                }
+                __syncthreads();
+                if (R + 1 < nt) fetch(R + 1);
+#pragma unroll
+                for (int s = 8 * h; s < 8 * h + 8; ++s) {
+                    const double a = lds[SB_LA + (16 * bi + (lane & 15)) * 65 + 4 * s + (lane >> 4)];
+                    const double b = lds[SB_LV + (16 * bj + (lane & 15)) * 65 + 4 * s + (lane >> 4)];
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+                }
+            }
+            __syncthreads();
+            if (h == 1) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) lds[SB_RED + (w & 7) * 256 + v * 64 + lane] = acc[v];
+            }
+            __syncthreads();
+            if (h == 0) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const double y = acc[v] + lds[SB_RED + (w & 7) * 256 + v * 64 + lane];
+                    const int rr = 64 * C + 16 * bi + 4 * v + (lane >> 4), kk = 16 * bj + (lane & 15);
+                    Yb[(size_t)rr * SBR_B + kk] = y;
+                }
+            }
+            __syncthreads();
+            // Gram partials of block C: G = V^T V, M = V^T Y over its rows
+            {
+                double *const Lv = lds + SB_LA;         // V[C] as [row][k] (64 x 33)
+                double *const Ly = lds + SB_LA + 64 * 33;  // Y[C] (64 x 33)
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int e = tid + TRI_NT * u, kk = e >> 6, rr = e & 63;
+                    Lv[rr * 33 + kk] = ld_dev(Vb + (size_t)kk * npad + 64 * C + rr);
+                    const int rr2 = e >> 5, kk2 = e & 31;
+                    Ly[rr2 * 33 + kk2] = Yb[(size_t)(64 * C + rr2) * SBR_B + kk2];
+                }
+                __syncthreads();
+                const int i = tid >> 5, jj = tid & 31;
+                double g = 0.0, mm = 0.0;
+                for (int rr = 0; rr < 64; ++rr) {
+                    const double vi = Lv[rr * 33 + i];
+                    g = fma(vi, Lv[rr * 33 + jj], g);
+                    mm = fma(vi, Ly[rr * 33 + jj], mm);
+                }
+                st_dev(gram + (size_t)C * 2 * SBR_B * SBR_B + tid, g);
+                st_dev(gram + (size_t)C * 2 * SBR_B * SBR_B + SBR_B * SBR_B + tid, mm);
+                __syncthreads();
+            }
+        }
+        SB_PROF(5);
+        ++phase;
+        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
+        SB_PROF(6);
+        // ---- G, M summed over the blocks: element e of the 2 x 32 x 32 by
+        // rank e % T (blocks B = Bt + g + 32 u by thread group g, the 32
+        // group sums added in order, so the bits do not depend on the rank),
+        // published -> barrier -> read by every rank
+        {
+            const int nme = (2 * SBR_B * SBR_B + T - 1 - rank) / T;  // elements of this rank
+            const int g = tid & 31;
+            for (int e0 = 0; e0 < nme; e0 += TRI_NT / 32) {
+                const int le = e0 + (tid >> 5), e = rank + T * min(le, nme - 1);
+                double t[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int B = min(Bt + g + 32 * u, nt - 1);
+                    t[u] = ld_dev(gram + (size_t)B * 2 * SBR_B * SBR_B + e);
+                }
+                double s = 0.0;
+#pragma unroll
+                for (int u = 0; u < 16; ++u) s += Bt + g + 32 * u < nt ? t[u] : 0.0;
+                lds[SB_PS + (tid >> 5) * 33 + g] = s;
+                __syncthreads();
+                if (g == 0 && le < nme) {
+                    double tt = 0.0;
+                    for (int gg = 0; gg < 32; ++gg) tt += lds[SB_PS + (tid >> 5) * 33 + gg];
+                    st_dev(gtot + e, tt);
+                }
+                __syncthreads();
+            }
+        }
+        ++phase;
+        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
+        SB_PROF(7);
+        // T (dlarft, forward columnwise); S = T^T M T; W = Y T - V S / 2
+        {
+            const int i = tid >> 5, jj = tid & 31;
+            lds[SB_G + i * 33 + jj] = ld_dev(gtot + tid);
+            lds[SB_M + i * 33 + jj] = ld_dev(gtot + SBR_B * SBR_B + tid);
+            lds[SB_T + i * 33 + jj] = 0.0;
+        }
+        __syncthreads();
+        for (int k = 0; k < SBR_B; ++k) {
+            const double tk = co[64 + k];
+            double t = 0.0;
+            if (tid < k)
+                for (int l = tid; l < k; ++l) t = fma(lds[SB_T + tid * 33 + l], lds[SB_G + l * 33 + k], t);
+            __syncthreads();
+            if (tid < k) lds[SB_T + tid * 33 + k] = -tk * t;
+            if (tid == k) lds[SB_T + k * 33 + k] = tk;
+            __syncthreads();
+        }
+        {
+            const int i = tid >> 5, jj = tid & 31;
+            double t = 0.0;  // (M T)[i][jj]
+            for (int l = 0; l < SBR_B; ++l) t = fma(lds[SB_M + i * 33 + l], lds[SB_T + l * 33 + jj], t);
+            lds[SB_S + i * 33 + jj] = t;  // temporarily M T
+        }
+        __syncthreads();
+        {
+            const int i = tid >> 5, jj = tid & 31;
+            double t = 0.0;  // (T^T (M T))[i][jj]
+            for (int l = 0; l < SBR_B; ++l) t = fma(lds[SB_T + l * 33 + i], lds[SB_S + l * 33 + jj], t);
+            __syncthreads();
+            lds[SB_S + i * 33 + jj] = t;
+        }
+        __syncthreads();
+        for (int q = 0; q < nq; ++q) {
+            const int C = sbr_qblock(Bt, nt, rank, T, q);
+            double *const Lv = lds + SB_LA;            // V[C] as [row][k] (64 x 33)
+            double *const Ly = lds + SB_LA + 64 * 33;  // Y[C] (64 x 33)
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int e = tid + TRI_NT * u, kk = e >> 6, rr = e & 63;
+                Lv[rr * 33 + kk] = ld_dev(Vb + (size_t)kk * npad + 64 * C + rr);
+                const int rr2 = e >> 5, kk2 = e & 31;
+                Ly[rr2 * 33 + kk2] = Yb[(size_t)(64 * C + rr2) * SBR_B + kk2];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int e = tid + TRI_NT * u, jj = e >> 6, rr = e & 63;  // W[64 C + rr][jj]
+                const int rw = 64 * C + rr;
+                double t1 = 0.0, t2 = 0.0;
+                for (int l = 0; l < SBR_B; ++l) {
+                    t1 = fma(Ly[rr * 33 + l], lds[SB_T + l * 33 + jj], t1);
+                    t2 = fma(Lv[rr * 33 + l], lds[SB_S + l * 33 + jj], t2);
+                }
+                const double x = (rw >= c0 && rw < n) ? t1 - 0.5 * t2 : 0.0;
+                st_dev(Wb + (size_t)jj * npad + rw, x);
+            }
+        }
+        SB_PROF(8);
+        ++phase;
+        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
+        SB_PROF(9);
+        // ---- A22 -= V W^T + W V^T on the owned tile rows
+        for (int q = 0; q < nq; ++q) {
+            const int C = sbr_qblock(Bt, nt, rank, T, q);
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 2 * SBR_B * 64 / TRI_NT; ++u) {
+                const int e = tid + TRI_NT * u, k = e >> 6, i = e & 63;
+                const int kk = k < SBR_B ? k : k - SBR_B;
+                const int rw = 64 * C + i;
+                lds[SB_LS + e] = -(k < SBR_B ? ld_dev(Vb + (size_t)kk * npad + rw) : ld_dev(Wb + (size_t)kk * npad + rw));
+            }
+            __syncthreads();
+            for (int R = Bt; R < nt; ++R) sbr_syr2k_tile(A, n, npad, C, R, c0, Vb, Wb);
+        }
+        SB_PROF(10);
+    }
+#ifdef NRG_SBR_PROF
+    if (rank == 0 && tid == 0)
+        printf("sbrprof n=%d T=%d mirror %lld qr %lld qrbar %lld qrupd %lld wb+bar %lld symm %lld bar %lld gram+bar %lld tsw %lld bar %lld syr2k %lld (x10ns)\n",
+               n, T, pacc[0], pacc[1], pacc[2], pacc[3], pacc[4], pacc[5], pacc[6], pacc[7], pacc[8], pacc[9], pacc[10]);
+#endif
+#undef SB_PROF
+}
+
 __device__ __forceinline__ void tri_teams(const EigMat *mats, const TriJob *jobs, const int *tbase, const int *qjob,
                                           int *counters, int *status) {
     __shared__ double red[TRI_NW];
@@ -1387,7 +1875,8 @@ __device__ __forceinline__ void tri_teams(const EigMat *mats, const TriJob *jobs
         const TriJob jb = jobs[lo];
         const EigMat mt = mats[jb.mat];
         const int rank = t0 + t - tbase[lo];
-        if (jb.blocked) tri_blk(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag);
+        if (jb.blocked == 2) sbr_blk(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag);
+        else if (jb.blocked) tri_blk(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag);
         else if (mt.n <= TEAM_LDS_N) tri_team<true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, red, &s_flag);
         else tri_team<false>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, red, &s_flag);
         __syncthreads();
@@ -1404,6 +1893,331 @@ __global__ __launch_bounds__(TRI_NT) void k_eig_tri_coop(const EigMat *mats, con
 __global__ __launch_bounds__(TRI_NT) void k_eig_tri_blk(const EigMat *mats, const TriJob *jobs, const int *tbase,
                                                         const int *qjob, int *counters, int *status) {
     tri_teams(mats, jobs, tbase, qjob, counters, status);
+}
+
+// ---------------------------------------------------------------------------
+// 1f. two-stage reduction, stage 2: band -> tridiagonal by bulge chasing.
+//     Sweep s (s = 0 .. n - 2) removes column s below the subdiagonal with a
+//     reflector on rows s + 1 .. s + b (task 0) and then chases the bulge:
+//     task k >= 1 removes the fill of column c = s + 1 + (k - 1) b with a
+//     reflector on rows r0 = s + 1 + k b .. r1 - 1 (r1 = min(r0 + b, n)),
+//     applied two-sided.  Task (s, k) reads and writes only columns [c, r1)
+//     of the lower band (offsets d < 2b below the diagonal), so sweep s may
+//     run task k once sweep s - 1 has finished its tasks 0 .. k + 2: sweeps
+//     run concurrently, one per chaser workgroup, each waiting on its
+//     predecessor's progress counter.  A chaser keeps the 2b columns of its
+//     current task in LDS (ring of 2b slots): task k loads the b new columns
+//     [r0, r1) and writes back the b columns [c, r0) it finishes with.
+//     Output: d, e (V_DG, V_OD); reflector (s, k) (tau in place of its unit
+//     first element) in row n - 1 - s of A's strict lower triangle, columns
+//     k b .. k b + len - 1 (the upper triangle holds stage 1's reflectors).
+// ---------------------------------------------------------------------------
+constexpr int CH_NT = 256;
+constexpr int CH_W = 2 * SBR_B;     // columns in a chaser's ring, entries per band column
+constexpr int CH_DONE = 1 << 30;    // progress of a finished sweep
+constexpr long long CH_SPIN_LIMIT = 1LL << 23;  // polls of ~1 us: seconds, far beyond any wait of a running team
+
+struct SbrJob {
+    int mat, P;  // matrix, chasers of its team
+};
+
+// band buffer in S1: column j holds A[j + d][j], d < CH_W (zero beyond n), then n progress ints
+__host__ __device__ inline double *sbr_band(double *ws, int n) {
+    return (double *)(((uintptr_t)(ws + (size_t)n * n) + 511) & ~(uintptr_t)511);
+}
+__host__ __device__ inline int *sbr_prog(double *ws, int n) { return (int *)(sbr_band(ws, n) + (size_t)n * CH_W); }
+
+// After stage 1: the band (row j, columns j .. j + SBR_B) into the band
+// buffer, taking a panel row's columns from the panel's trailing start on
+// from S2; the panel rows' R and reflectors copied from S2 into A (S2 is
+// the divide-and-conquer workspace later); progress counters zeroed.
+// Item: (matrix, 16 rows).
+__global__ __launch_bounds__(256) void k_sbr_band(const EigMat *mats, const BtItem *items) {
+    const BtItem it = items[blockIdx.x];
+    const EigMat mt = mats[it.mat];
+    const int n = mt.n, nr = sbr_nrefl(n);
+    double *band = sbr_band(mt.ws, n);
+    int *prog = sbr_prog(mt.ws, n);
+    const double *P2 = mt.ws + 2 * (size_t)n * n;
+    const int j0 = it.c0;
+    for (int e = threadIdx.x; e < 16 * CH_W; e += 256) {  // 16 rows per item
+        const int j = j0 + e / CH_W, d = e % CH_W;
+        if (j >= n) continue;
+        const int c0j = (j / SBR_B + 1) * SBR_B;  // trailing start of row j's panel
+        const int col = j + d;
+        double v = 0.0;
+        if (d <= SBR_B && col < n) v = (j < nr && col >= c0j) ? P2[(size_t)j * n + col] : mt.A[(size_t)j * n + col];
+        band[(size_t)j * CH_W + d] = v;
+        if (d == 0) prog[j] = 0;
+    }
+    for (int jj = 0; jj < 16; ++jj) {
+        const int j = j0 + jj;
+        if (j >= nr) break;
+        const int c0j = (j / SBR_B + 1) * SBR_B;
+        for (int col = c0j + threadIdx.x; col < n; col += 256) mt.A[(size_t)j * n + col] = P2[(size_t)j * n + col];
+    }
+}
+
+__device__ __forceinline__ bool ch_wait(const int *prog, int need, int *status, int *s_flag) {
+    if (threadIdx.x == 0) {
+        long long spins = 0;
+        int ok = 1;
+        while (__hip_atomic_load(prog, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > CH_SPIN_LIMIT || __hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ok = 0;
+                atomicCAS(status, 0, NRG_ERR_HIP);
+                break;
+            }
+        }
+        *s_flag = ok;
+    }
+    __syncthreads();
+    return *s_flag != 0;
+}
+
+// one sweep of one chaser
+__device__ __forceinline__ bool ch_sweep(const EigMat &mt, int s, double (*ring)[CH_W + 1], double *vs, double *dl,
+                                         double *wq, double *db, double *sc, int *status, int *s_flag) {
+    const int n = mt.n, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    double *const band = sbr_band(mt.ws, n);
+    int *const prog = sbr_prog(mt.ws, n);
+    double *const gdg = vslot(mt.ws, n, V_DG), *const god = vslot(mt.ws, n, V_OD);
+    auto slot = [&](int j) { return (j - s - 1 + CH_W) % CH_W; };
+    auto L = [&](int j, int d) -> double & { return ring[slot(j)][d]; };
+    auto load_cols = [&](int ja, int jb) {
+        for (int e = tid; e < (jb - ja) * CH_W; e += CH_NT) {
+            const int j = ja + e / CH_W, d = e % CH_W;
+            ring[slot(j)][d] = ld_dev(band + (size_t)j * CH_W + d);
+        }
+    };
+    auto store_cols = [&](int ja, int jb) {
+        for (int e = tid; e < (jb - ja) * CH_W; e += CH_NT) {
+            const int j = ja + e / CH_W, d = e % CH_W;
+            st_dev(band + (size_t)j * CH_W + d, ring[slot(j)][d]);
+        }
+    };
+    auto signal = [&](int v) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) __hip_atomic_store(prog + s, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    if (s > 0 && !ch_wait(prog + s - 1, 3, status, s_flag)) return false;
+    load_cols(s, min(s + SBR_B + 1, n));
+    __syncthreads();
+    for (int k = 0;; ++k) {
+        const int c = k == 0 ? s : s + 1 + (k - 1) * SBR_B;
+        const int r0 = s + 1 + k * SBR_B, r1 = min(r0 + SBR_B, n), len = r1 - r0;
+        const int r2 = min(r1 + SBR_B, n), nb = r2 - r1;
+        if (k > 0) {
+            if (s > 0 && !ch_wait(prog + s - 1, k + 3, status, s_flag)) return false;
+            load_cols(r0, r1);
+            __syncthreads();
+        }
+        // reflector from column c, rows r0 .. r1 - 1
+        if (w == 0) {
+            const double x = lane < len ? L(c, r0 - c + lane) : 0.0;
+            const double sigma = wave_sum(lane >= 1 && lane < len ? x * x : 0.0);
+            const double alpha = readlane_d(x, 0);
+            double tau = 0.0, beta = alpha, scale = 0.0;
+            if (sigma != 0.0) {
+                beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+                tau = (beta - alpha) / beta;
+                scale = 1.0 / (alpha - beta);
+            }
+            if (lane < SBR_B) vs[lane] = lane == 0 ? 1.0 : lane < len ? x * scale : 0.0;
+            if (lane < len) {
+                L(c, r0 - c + lane) = lane == 0 ? beta : 0.0;
+                // reflector (s, k): row n - 1 - s, columns k b ..
+                mt.A[(size_t)(n - 1 - s) * n + k * SBR_B + lane] = lane == 0 ? tau : x * scale;
+            }
+            if (lane == 0) {
+                sc[0] = tau;
+                if (k == 0) {
+                    gdg[s] = L(s, 0);
+                    god[s] = beta;
+                }
+            }
+        }
+        __syncthreads();
+        const double tau = sc[0];
+        if (tau != 0.0) {
+            // dot products: left columns (c, r0), D v, below rows [r1, r2)
+            const int nl = r0 - c - 1;
+            if (tid < SBR_B) {
+                if (tid < nl) {
+                    const int j = c + 1 + tid;
+                    double t = 0.0;
+                    for (int u = 0; u < len; ++u) t = fma(vs[u], L(j, r0 + u - j), t);
+                    dl[tid] = t;
+                }
+            } else if (tid < 2 * SBR_B) {
+                const int t = tid - SBR_B;
+                if (t < len) {
+                    double a = 0.0;
+                    for (int u = 0; u < len; ++u) a = fma(t >= u ? L(r0 + u, t - u) : L(r0 + t, u - t), vs[u], a);
+                    wq[t] = tau * a;  // p = tau D v
+                }
+            } else if (tid < 3 * SBR_B) {
+                const int t = tid - 2 * SBR_B;
+                if (t < nb) {
+                    double a = 0.0;
+                    for (int u = 0; u < len; ++u) a = fma(L(r0 + u, r1 + t - r0 - u), vs[u], a);
+                    db[t] = a;
+                }
+            }
+            __syncthreads();
+            if (w == 0) {  // q = p - (tau / 2) (v . p) v
+                const double pv = wave_sum(lane < len ? vs[lane] * wq[lane] : 0.0);
+                const double K = 0.5 * tau * pv;
+                if (lane < len) wq[lane] -= K * vs[lane];
+            }
+            __syncthreads();
+            const int nL = nl * len, nD = len * len, nB = nb * len;
+            for (int e = tid; e < nL + nD + nB; e += CH_NT) {
+                if (e < nL) {
+                    const int jj = e / len, u = e % len, j = c + 1 + jj;
+                    L(j, r0 + u - j) -= tau * dl[jj] * vs[u];
+                } else if (e < nL + nD) {
+                    const int f = e - nL, t = f / len, u = f % len;
+                    if (t >= u) L(r0 + u, t - u) -= vs[t] * wq[u] + wq[t] * vs[u];
+                } else {
+                    const int f = e - nL - nD, t = f / len, u = f % len;
+                    L(r0 + u, r1 + t - r0 - u) -= tau * db[t] * vs[u];
+                }
+            }
+            __syncthreads();
+        }
+        if (r1 >= n) {  // last task: the whole window goes back
+            if (s == n - 2 && tid == 0) {
+                gdg[n - 1] = L(n - 1, 0);
+                god[n - 1] = 0.0;
+            }
+            store_cols(c, n);
+            signal(CH_DONE);
+            return true;
+        }
+        store_cols(c, r0);
+        signal(k + 1);
+    }
+}
+
+__global__ __launch_bounds__(CH_NT) void k_sbr_chase(const EigMat *mats, const SbrJob *jobs, const int *tbase, int njobs,
+                                                     int *ticket, int *status) {
+    __shared__ double ring[CH_W][CH_W + 1];
+    __shared__ double vs[SBR_B], dl[SBR_B], wq[SBR_B], db[SBR_B], sc[4];
+    __shared__ int s_t, s_flag;
+    const int total = tbase[njobs];
+    for (;;) {
+        if (threadIdx.x == 0) s_t = atomicAdd(ticket, 1);
+        __syncthreads();
+        const int t = s_t;
+        __syncthreads();
+        if (t >= total) break;
+        int lo = 0, hi = njobs;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (tbase[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const SbrJob jb = jobs[lo];
+        const EigMat mt = mats[jb.mat];
+        const int q = t - tbase[lo];
+        for (int s = q; s < mt.n - 1; s += jb.P)
+            if (!ch_sweep(mt, s, ring, vs, dl, wq, db, sc, status, &s_flag)) return;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 1g. back-transformation of stage 2: x = Q2 y for every eigenvector y of the
+//     tridiagonal (row of Vc).  Q2 = prod over sweep blocks B = [s0, s0 + L)
+//     (ascending) of prod_{k descending} prod_{s in B ascending} H(s, k): the
+//     same product as the sweeps' order, since H(s, k) and H(s', k') with
+//     s < s' commute whenever k' > k and reflectors of one sweep are
+//     disjoint.  So for each block (the last first) and k = 0, 1, .. the
+//     reflectors H(s, k), s descending, act on the window [s0 + 1 + k b,
+//     s0 + 1 + k b + b + L - 1) of every vector.  Lane = vector: the window
+//     lives in registers (compile-time indices), moves by b per step (b
+//     finished entries stored, b new ones loaded); the step's reflectors are
+//     staged in LDS and read by all lanes alike.  BL sweeps per block: the
+//     vectors are streamed through once per block.
+// ---------------------------------------------------------------------------
+// One wavefront per 64 vectors (lane = vector), BL sweeps per block; the
+// next step's new window entries are loaded while this step computes.
+// (Measured on the keep-10000 block set: BL = 48 1.14 s, 32 1.30 s, 64
+// with spills 1.60 s; LDS-transposed coalesced window traffic 1.59 s and
+// four wavefronts sharing a staging 1.68 s: the reflectors' LDS broadcast
+// reads, not HBM, bound the step.)
+constexpr int SBT_BL = 48;  // sweeps per block of the stage-2 back-transformation
+template <int BL>
+__global__ __launch_bounds__(64) void k_sbr_bt(const EigMat *mats, const BtItem *items) {
+    constexpr int WIN = SBR_B + BL - 1;  // window length
+    __shared__ double rv[BL][SBR_B];
+    __shared__ double rt[BL];
+    const BtItem it = items[blockIdx.x];
+    const EigMat mt = mats[it.mat];
+    const int n = mt.n, lane = threadIdx.x;
+    const int cv = it.c0 + lane;  // vector (row of Vc)
+    const bool act = cv < n;
+    double *const zrow = mt.ws + (size_t)(act ? cv : 0) * n;
+    const double *const A = mt.A;
+    const int nsw = n - 1;  // sweeps 0 .. n - 2
+    const int nblk = (nsw + BL - 1) / BL;
+    double z[WIN], zn[SBR_B];
+    for (int bi = nblk - 1; bi >= 0; --bi) {
+        const int s0 = bi * BL, s1 = min(s0 + BL, nsw);
+        int base = s0 + 1;
+#pragma unroll
+        for (int p = 0; p < WIN; ++p) z[p] = (act && base + p < n) ? zrow[base + p] : 0.0;
+        for (int k = 0; base < n; ++k, base += SBR_B) {
+            const bool more = base + SBR_B < n;
+#pragma unroll
+            for (int p = 0; p < SBR_B; ++p) {
+                const int idx = base + WIN + p;
+                zn[p] = (act && more && idx < n) ? zrow[idx] : 0.0;
+            }
+            // reflectors H(s, k), s in [s0, s1): row n-1-s, columns k b ..
+            for (int e = lane; e < BL * SBR_B; e += 64) {
+                const int o = e / SBR_B, t = e % SBR_B, s = s0 + o;
+                const int st = base + o;  // first index of H(s, k)
+                double v = 0.0;
+                if (s < s1 && st < n && st + t < n) v = A[(size_t)(n - 1 - s) * n + k * SBR_B + t];
+                if (t == 0) {
+                    rt[o] = (s < s1 && st < n) ? v : 0.0;
+                    v = 1.0;
+                }
+                rv[o][t] = v;
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int o = BL - 1; o >= 0; --o) {
+                const double tau = rt[o];
+                if (tau == 0.0) continue;
+                // four partial sums: a chain of 8 dependent FMAs instead of 32
+                double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+#pragma unroll
+                for (int t = 0; t < SBR_B; t += 4) {
+                    d0 = fma(rv[o][t], z[o + t], d0);
+                    d1 = fma(rv[o][t + 1], z[o + t + 1], d1);
+                    d2 = fma(rv[o][t + 2], z[o + t + 2], d2);
+                    d3 = fma(rv[o][t + 3], z[o + t + 3], d3);
+                }
+                const double d = tau * ((d0 + d1) + (d2 + d3));
+#pragma unroll
+                for (int t = 0; t < SBR_B; ++t) z[o + t] = fma(-d, rv[o][t], z[o + t]);
+            }
+            wave_lds_sync();
+            // entries base .. base + b - 1 are final for this block
+#pragma unroll
+            for (int p = 0; p < SBR_B; ++p)
+                if (act && base + p < n) zrow[base + p] = z[p];
+#pragma unroll
+            for (int p = 0; p < WIN - SBR_B; ++p) z[p] = z[p + SBR_B];
+#pragma unroll
+            for (int p = 0; p < SBR_B; ++p) z[WIN - SBR_B + p] = zn[p];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1427,7 +2241,7 @@ __global__ __launch_bounds__(256) void k_dc_leaf(const EigMat *mats) {
     const double *dg = vslot(ws, n, V_DG), *od = vslot(ws, n, V_OD);
     double *D = vslot(ws, n, V_D), *beta = vslot(ws, n, V_BETA);
     double *Vc = ws;
-    if (mt.pad) {
+    if (mt.pad == 1) {
         // reflectors of a team reduction: S2 row j, columns j+1.. -> row j of A
         const double *Y = ws + 2 * (size_t)n * n;
         for (int j = 0; j < n - 2; ++j)
@@ -2552,7 +3366,8 @@ struct WyItem {
 };
 
 // Y^T: row j = reflector j (columns j+1.., first element included), zero
-// elsewhere; rows n-2, n-1 are zero.
+// elsewhere; rows n-2, n-1 are zero.  Two-stage matrices: reflector j of
+// stage 1 at columns j + SBR_B.. (its unit first element written here).
 __global__ __launch_bounds__(256) void k_wy_extract(const EigMat *mats, const BtItem *items) {
     const BtItem it = items[blockIdx.x];
     const EigMat mt = mats[it.mat];
@@ -2561,6 +3376,12 @@ __global__ __launch_bounds__(256) void k_wy_extract(const EigMat *mats, const Bt
     const int j = it.c0;
     const double *src = mt.A + (size_t)j * n;
     double *dst = Yt + (size_t)j * n;
+    if (mt.pad == 2) {  // two-stage: v_j at rows j + SBR_B.. (unit first element)
+        const bool on = j < sbr_nrefl(n);
+        for (int r = threadIdx.x; r < n; r += 256)
+            dst[r] = (on && r >= j + SBR_B) ? (r == j + SBR_B ? 1.0 : src[r]) : 0.0;
+        return;
+    }
     for (int r = threadIdx.x; r < n; r += 256) dst[r] = (j <= n - 3 && r > j) ? src[r] : 0.0;
 }
 
@@ -2700,6 +3521,12 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                                       (int)TRI_COOP_LDS));
         if (const char *e = getenv("NRG_BLK_MIN_N")) c->blk_min_n = std::max(3, atoi(e));
         if (const char *e = getenv("NRG_BLK_TDIV")) c->blk_tdiv = std::max(0, atoi(e));
+        if (const char *e = getenv("NRG_SBR")) c->sbr = atoi(e) != 0;
+        {
+            int per_cu_ch = 0;
+            HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_ch, k_sbr_chase, CH_NT, 0));
+            c->chase_capacity = std::max(1, per_cu_ch * ncu);
+        }
         HIPCHK(c, hipFuncSetAttribute((const void *)k_eig_tri_blk, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)TRI_COOP_LDS));
         HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_eig_tri_coop, TRI_NT, TRI_COOP_LDS));
@@ -2739,6 +3566,12 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             cost[i] = 2.0 * n;
             continue;
         }
+        if (tasks[i].n >= blk_min_n && c->sbr && sbr_ws_doubles(tasks[i].n) <= (size_t)tasks[i].n * tasks[i].n &&
+            (size_t)tasks[i].n * (CH_W + 1) + 64 <= (size_t)tasks[i].n * tasks[i].n) {
+            blkjob[i] = 2;  // two-stage; team size below, once all blocked jobs are known
+            mats[i].pad = 2;
+            continue;
+        }
         if (tasks[i].n >= blk_min_n && blk_ws_doubles(tasks[i].n) <= (size_t)tasks[i].n * tasks[i].n) {
             blkjob[i] = 1;  // team size below, once all blocked jobs are known
             continue;
@@ -2770,8 +3603,15 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     // the queues run them in turn.  NRG_BLK_TDIV > 0 instead gives every
     // matrix nt / NRG_BLK_TDIV ranks.
     {
-        auto blk_o = [](double n) { return n * std::max(0.0114 * n - 25.0, 10.0); };
-        auto blk_w = [](double n) { return 0.266 * n * n + 7e-5 * n * n * n; };
+        // The two-stage reduction's stage 1 (blkjob 2) pays ~34 barriers per
+        // panel of 32 columns and moves its work into FP64-MFMA tile
+        // products (2 n^3 flops, ~0.15 TF per CU): o(n) ~ 3.5 n us,
+        // W(n) ~ 1.4e-5 n^3 CU-us.
+        auto blk_o1 = [](double n) { return n * std::max(0.0114 * n - 25.0, 10.0); };
+        auto blk_w1 = [](double n) { return 0.266 * n * n + 7e-5 * n * n * n; };
+        const bool two = c->sbr != 0;
+        auto blk_o = [&](double n) { return two ? 3.5 * n : blk_o1(n); };
+        auto blk_w = [&](double n) { return two ? 0.5 * n * n + 1.4e-5 * n * n * n : blk_w1(n); };
         const int tmax = std::min(TEAM_GLOBAL_MAX, tri_grid);
         // between one rank per four and one per two 64-column blocks: smaller
         // teams measured slower than the model in a full mix (their ranks'
@@ -2806,6 +3646,11 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             const int nt = (tasks[i].n + 63) / 64;
             int T = blk_tdiv > 0 ? std::max(2, nt / blk_tdiv) : team_for(n, M);
             T = std::min(T, tmax);
+            if (blkjob[i] == 2) {  // a rank holds at most SBR_MAXQ blocks of panel rows
+                while (T < tmax && sbr_owned_max(nt, T) > SBR_MAXQ) ++T;
+                if (sbr_owned_max(nt, T) > SBR_MAXQ)
+                    return fail(c, NRG_ERR_ARG, "diag: two-stage team too small for dimension " + std::to_string(tasks[i].n));
+            }
             if (T > std::min(TEAM_MAX, per_xcd)) global[i] = 1;
             teamT[i] = T;
             cost[i] = blk_o(n) + blk_w(n) / T;
@@ -2859,8 +3704,9 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     // counters: tickets + one barrier per job, register teams first; barrier
     // counters a cache line apart (teams polling neighbouring counters would
     // contend for one line)
+    // (the last counter: the bulge chasers' ticket)
     const size_t creg = N_XCD + 1 + (size_t)BAR_STRIDE * rjobs.size(),
-                 ncnt = creg + N_XCD + 1 + (size_t)BAR_STRIDE * jobs.size();
+                 ncnt = creg + N_XCD + 1 + (size_t)BAR_STRIDE * jobs.size() + 1;
     if (ncnt > c->cnt_ints) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (c->cnt) HIPCHK(c, hipFree(c->cnt));
@@ -2872,8 +3718,22 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     // plan is built, so the host builds that while it runs.  Its slot of the
     // staging ring (the matrix descriptors, which the later stages read too)
     // is next written by a copy queued on this stream after all of them.
+    // two-stage matrices: band extraction items (16 columns each) and the
+    // chaser teams (largest first; a team never exceeds the co-resident grid)
+    std::vector<BtItem> sbitems;
+    std::vector<SbrJob> sjobs;
+    std::vector<int> stbase(1, 0);
+    for (int i = 0; i < ntasks; ++i) {
+        if (blkjob[i] != 2) continue;
+        const int n = tasks[i].n;
+        for (int j = 0; j < n; j += 16) sbitems.push_back({i, j});
+        const int P = std::max(1, std::min((n - 1) / (3 * SBR_B) + 1, c->chase_capacity));
+        sjobs.push_back({i, P});
+        stbase.push_back(stbase.back() + P);
+    }
     HostArena ar1;
     const size_t o_jobs = ar1.put(jobs), o_tb = ar1.put(tbase), o_qj = ar1.put(qjob), o_ones = ar1.put(ones);
+    const size_t o_sbitems = ar1.put(sbitems), o_sjobs = ar1.put(sjobs), o_stbase = ar1.put(stbase);
     const size_t o_rjobs = ar1.put(rjobs), o_rtb = ar1.put(rtbase), o_rqj = ar1.put(rqjob);
     const size_t o_mats = ar1.put(mats);
     void *d1;
@@ -2899,8 +3759,10 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             double &fl = tasks[i].n <= REG_ONE_N ? fl1 : regjob[i] ? flr : flt;
             double &by = tasks[i].n <= REG_ONE_N ? by1 : regjob[i] ? byr : byt;
             fl += 4.0 / 3.0 * n * n * n;
-            // the blocked product reads half the trailing matrix per column
-            by += (blkjob[i] ? 4.0 : 8.0) / 3.0 * n * n * n;
+            // the blocked product reads half the trailing matrix per column;
+            // the two-stage reduction reads and writes the trailing matrix a
+            // few times per panel of 32 columns (3 passes: 8 n^3 / 32 bytes)
+            by += (blkjob[i] == 2 ? 0.25 : blkjob[i] ? 4.0 / 3.0 : 8.0 / 3.0) * n * n * n;
         }
         bool any_blk = false;
         for (int i = 0; i < ntasks; ++i) any_blk = any_blk || blkjob[i];
@@ -2938,6 +3800,21 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             }
             HIPCHK(c, hipEventRecord(c->ev_join, c->side));
             HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+        }
+        // two-stage matrices: band -> tridiagonal by bulge chasing (grid <=
+        // co-resident capacity; every team fits it, so every chaser's
+        // predecessor runs)
+        if (!sjobs.empty()) {
+            hipLaunchKernelGGL(k_sbr_band, dim3(sbitems.size()), dim3(256), 0, c->stream, d_mats,
+                               (const BtItem *)(db1 + o_sbitems));
+            HIPCHK(c, hipGetLastError());
+            double flc = 0.0;
+            for (const SbrJob &jb : sjobs) flc += 6.0 * SBR_B * (double)tasks[jb.mat].n * tasks[jb.mat].n;
+            LaunchTimer lt(K_SBR_CHASE, c->stream, flc, 0.0);
+            const int grid = std::min(c->chase_capacity, stbase.back());
+            hipLaunchKernelGGL(k_sbr_chase, dim3(grid), dim3(CH_NT), 0, c->stream, d_mats,
+                               (const SbrJob *)(db1 + o_sjobs), (const int *)(db1 + o_stbase), (int)sjobs.size(),
+                               c->cnt + ncnt - 1, c->dstatus);
         }
     }
     HIPCHK(c, hipGetLastError());
@@ -3028,10 +3905,16 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     std::vector<BtItem> bt[kBtVariants];
     std::vector<BtItem> rk, cp, wyrows;  // wyrows: one row per item (extract, permuted copy)
     std::vector<int> wymat;              // matrices back-transformed by WY blocks
+    std::vector<BtItem> sbt;             // two-stage: 64 eigenvectors per item (stage 2's reflectors)
     int wy_blocks = 0;
     for (int i = 0; i < nbig; ++i) {
         const int n = tasks[i].n, ns = (n + 63) / 64;
-        if (n >= BT_WY_MIN) {
+        if (mats[i].pad == 2) {
+            for (int c0 = 0; c0 < n; c0 += 64) sbt.push_back({i, c0});
+            wymat.push_back(i);
+            wy_blocks = std::max(wy_blocks, (sbr_nrefl(n) + WY_NB - 1) / WY_NB);
+            for (int r = 0; r < n; ++r) wyrows.push_back({i, r});
+        } else if (n >= BT_WY_MIN) {
             wymat.push_back(i);
             wy_blocks = std::max(wy_blocks, (n - 2 + WY_NB - 1) / WY_NB);
             for (int r = 0; r < n; ++r) wyrows.push_back({i, r});
@@ -3045,7 +3928,9 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         for (int c0 = 0; c0 < n; c0 += 64) rk.push_back({i, c0});
     }
     // WY blocks, counted from the last: block q of matrix i covers reflectors
-    // [max(0, j1 - WY_NB), j1) with j1 = n - 2 - q WY_NB
+    // [max(0, j1 - WY_NB), j1) with j1 = NR - q WY_NB: NR = n - 2 reflectors
+    // starting at row j + 1 (one-stage), or sbr_nrefl(n) starting at row
+    // j + SBR_B (two-stage; WyItem.pad holds that offset)
     std::vector<WyItem> wyitems;
     std::vector<nrg_gemm_out> wyouts;
     std::vector<nrg_gemm_term> wyterms;
@@ -3064,10 +3949,11 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         L.w0 = (int)wyitems.size();
         std::vector<int> act;
         for (int i : wymat) {
-            const int n = tasks[i].n, j1 = n - 2 - q * WY_NB;
+            const int n = tasks[i].n, two = mats[i].pad == 2;
+            const int j1 = (two ? sbr_nrefl(n) : n - 2) - q * WY_NB;
             if (j1 <= 0) continue;
             const int j0 = std::max(0, j1 - WY_NB);
-            wyitems.push_back({c->ews + off[i], n, j0, j1 - j0, 0});
+            wyitems.push_back({c->ews + off[i], n, j0, j1 - j0, two ? SBR_B : 1});
             act.push_back(i);
         }
         L.w1 = (int)wyitems.size();
@@ -3076,10 +3962,10 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             const size_t so0 = wyouts.size();
             for (int w = L.w0; w < L.w1; ++w) {
                 const WyItem &it = wyitems[w];
-                const int n = it.n, j0 = it.j0, nb = it.nb, np = n - j0 - 1;
+                const int n = it.n, j0 = it.j0, nb = it.nb, ro = it.pad, np = n - j0 - ro;
                 double *ws = it.ws, *Yt = ws + (size_t)n * n, *S2 = ws + 2 * (size_t)n * n;
                 double *G = S2, *T = S2 + WY_NB * WY_NB, *M = S2 + 2 * WY_NB * WY_NB, *W = M + (size_t)WY_NB * n;
-                const double *Yb = Yt + (size_t)j0 * n + j0 + 1;  // block rows, columns j0+1..
+                const double *Yb = Yt + (size_t)j0 * n + j0 + ro;  // block rows, columns j0+ro..
                 nrg_gemm_out o{};
                 nrg_gemm_term t{};
                 t.alpha = 1.0;
@@ -3089,11 +3975,11 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                 } else if (stage == 1) {  // M = T Yb
                     o.C = M; o.m = nb; o.n = np; o.ldc = n;
                     t.A = T; t.lda = WY_NB; t.B = Yb; t.ldb = n; t.k = nb;
-                } else if (stage == 2) {  // W = Z[:, j0+1:] M^T
+                } else if (stage == 2) {  // W = Z[:, j0+ro:] M^T
                     o.C = W; o.m = n; o.n = nb; o.ldc = WY_NB;
-                    t.A = ws + j0 + 1; t.lda = n; t.B = M; t.ldb = n; t.transB = 1; t.k = np;
-                } else {  // Z[:, j0+1:] -= W Yb
-                    o.C = ws + j0 + 1; o.m = n; o.n = np; o.ldc = n; o.beta = 1.0;
+                    t.A = ws + j0 + ro; t.lda = n; t.B = M; t.ldb = n; t.transB = 1; t.k = np;
+                } else {  // Z[:, j0+ro:] -= W Yb
+                    o.C = ws + j0 + ro; o.m = n; o.n = np; o.ldc = n; o.beta = 1.0;
                     t.A = W; t.lda = WY_NB; t.B = Yb; t.ldb = n; t.k = nb; t.alpha = -1.0;
                 }
                 o.term_begin = (int)wyterms.size();
@@ -3119,6 +4005,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                  o_cp = ar.put(cp);
     size_t o_bt[kBtVariants];
     for (int v = 0; v < kBtVariants; ++v) o_bt[v] = ar.put(bt[v]);
+    const size_t o_sbt = ar.put(sbt);
     const size_t o_wyrows = ar.put(wyrows), o_wyitems = ar.put(wyitems), o_wyouts = ar.put(wyouts),
                  o_wyterms = ar.put(wyterms), o_wygi = ar.put(wygi), o_wyred = ar.put(wyred);
     if (gslots) {
@@ -3251,6 +4138,14 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             if (tasks[i].n < BT_WY_MIN) by += 16.0 * tasks[i].n * tasks[i].n;
         LaunchTimer lt(K_EIG_COPY, c->stream, 0.0, by);
         hipLaunchKernelGGL(k_eig_copy, dim3(cp.size()), dim3(256), 0, c->stream, d_mats, (const BtItem *)(db + o_cp));
+    }
+    HIPCHK(c, hipGetLastError());
+    if (!sbt.empty()) {  // stage 2's reflectors on the eigenvectors of the tridiagonal
+        double fl = 0.0;
+        for (int i = 0; i < nbig; ++i)
+            if (mats[i].pad == 2) fl += 2.0 * (double)tasks[i].n * tasks[i].n * tasks[i].n;
+        LaunchTimer lt(K_SBR_BT, c->stream, fl, 0.0);
+        hipLaunchKernelGGL(k_sbr_bt<SBT_BL>, dim3(sbt.size()), dim3(64), 0, c->stream, d_mats, (const BtItem *)(db + o_sbt));
     }
     HIPCHK(c, hipGetLastError());
     if (!wymat.empty()) {

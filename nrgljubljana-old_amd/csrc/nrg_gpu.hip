@@ -19,7 +19,8 @@
 
 const char *kKernelNames[K_COUNT] = {"k_zero",       "k_assemble", "k_eig_tri_coop", "k_eig_tri_one", "k_eig_tri_reg", "k_eig_tri_blk", "k_dc_leaf", "k_dc_small", "k_dc_deflate",
                                      "k_dc_secular", "k_dc_zhat",  "k_dc_unorm", "k_gemm_dc", "k_gemm_bt", "k_dc_rank",
-                                     "k_eig_bt",     "k_eig_copy", "k_gemm",     "k_spec_bin", "k_gather_scatter"};
+                                     "k_eig_bt",     "k_eig_copy", "k_gemm",     "k_spec_bin", "k_gather_scatter",
+                                     "k_sbr_chase", "k_sbr_bt"};
 bool g_timing = false;
 int g_timing_only = -1;
 nrg_kernel_stat g_stats[K_COUNT];

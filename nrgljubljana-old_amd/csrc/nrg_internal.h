@@ -46,6 +46,11 @@ struct nrg_ctx {
     // blocked teams: sized by the makespan model (0), or one workgroup per
     // blk_tdiv 64-column blocks
     int blk_tdiv = 0;
+    // large matrices: two-stage reduction (band, then bulge chasing) when
+    // set, else the one-stage blocked reduction
+    int sbr = 1;
+    // co-resident bulge-chasing workgroups
+    int chase_capacity = 0;
     // grouped GEMM: partial tiles of split outputs
     double *gpart = nullptr;
     size_t gpart_tiles = 0;
@@ -105,6 +110,8 @@ enum KernelId {
     K_GEMM,
     K_SPEC,
     K_XFER,
+    K_SBR_CHASE,
+    K_SBR_BT,
     K_COUNT
 };
 extern const char *kKernelNames[K_COUNT];
