@@ -1407,9 +1407,9 @@ __host__ __device__ inline int sbr_owned_max(int nt, int T) {
     return mx;
 }
 // LDS map (doubles)
-constexpr int SB_LA = 0;                      // tile (64 x 65) of the product
-constexpr int SB_LV = SB_LA + 64 * 65;        // V of the tile's columns (32 x 65)
-constexpr int SB_RED = SB_LV + 32 * 65;       // halves of the product (8 x 256)
+constexpr int SB_LA = 0;                      // product tiles (2 x 64 x 65)
+constexpr int SB_LV = SB_LA + 2 * 64 * 65;    // V of the tiles' columns (2 x 32 x 65)
+constexpr int SB_RED = SB_LV + 2 * 32 * 65;   // halves of the product (8 x 256)
 constexpr int SB_T = SB_RED + 8 * 256;        // T, S, G, M (32 x 33 each)
 constexpr int SB_S = SB_T + 32 * 33;
 constexpr int SB_G = SB_S + 32 * 33;
@@ -1418,50 +1418,174 @@ constexpr int SB_CO = SB_M + 32 * 33;         // per-reflector coefficients, tau
 constexpr int SB_PS = SB_CO + 128;            // partial-sum chunks (32 x 33)
 constexpr int SB_TOT = SB_PS + 32 * 33;          // their totals
 constexpr int SB_END = SB_TOT + 32;
-constexpr int SB_LS = 0, SB_RS = 64 * 64;     // rank-2k operands (64 x 64 each, k-major)
-static_assert(SB_RS + 64 * 64 <= SB_T, "rank-2k operands overlap only the product's buffers");
+constexpr int SB_LS = 0, SB_RS = 64 * 64;     // rank-2k operands: rows (64 x 64), columns (2 x 64 x 64), k-major
+static_assert(SB_RS + 2 * 64 * 64 <= SB_T, "rank-2k operands overlap only the product's buffers");
 static_assert(sizeof(double) * SB_END <= TRI_COOP_LDS, "stage 1 fits the team launch's LDS");
 
-// A[r][c] -= sum_k V[r][k] W[c][k] + W[r][k] V[c][k] on tile (C rows, R
-// columns), rows and columns >= c0.  Ls (row operands -[V W] of block C) is
-// staged by the caller; Rs ([W V] at columns R) here.  Wavefront w computes
-// the 16 x 16 block (w / 4, w % 4), the old tile as the accumulator.
-__device__ __forceinline__ void sbr_syr2k_tile(double *__restrict__ A, int n, int npad, int C, int R, int c0,
-                                               const double *Vb, const double *Wb) {
-    extern __shared__ double lds[];
-    double *const Ls = lds + SB_LS, *const Rs = lds + SB_RS;
-    const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-    const int bi = w >> 2, bj = w & 3;
-    const int col = 64 * R + 16 * bj + (lane & 15);
-    const int row0 = 64 * C + 16 * bi + (lane >> 4);  // element v: row row0 + 4 v
-    blk_d4 acc;
-    gdouble *Ag = (gdouble *)A;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const double x = Ag[(size_t)min(row0 + 4 * v, n - 1) * n + min(col, n - 1)];
-        acc[v] = (row0 + 4 * v < n && col < n) ? x : 0.0;
+// The tile loops of the product and the rank-2k update walk the trailing
+// tile row of an owned block C: tile R's operands are loaded three tiles
+// ahead into a four-deep register ring, staged in one of two LDS buffers, and
+// one barrier per tile separates the staging from the MFMAs; so about three
+// tiles of loads are in flight while a tile is computed.
+#define SB_RING4(STEP)                          \
+    for (int i0_ = 0; i0_ < cnt; i0_ += 4) {    \
+        STEP(0) STEP(1) STEP(2) STEP(3)         \
     }
+
+// Y[C] = sum_R A[C rows][R columns] V[R] over R in [Bt, nt) (64 x 32): wavefront
+// w computes the 16 x 16 block (w & 3, (w >> 2) & 1) over half w >> 3 of each
+// tile's 64 columns (v_mfma_f64_16x16x4f64: A[i][k] in lane i + 16 k, B[k][j]
+// in lane j + 16 k, D[i][j] in lane j + 16 (i % 4), element i / 4); the
+// halves are added in order.  Rows of C are the rank's own (plain loads).
+__device__ __noinline__ void sbr_symm_row(const double *A_, int n_, int npad_, int nt_, int C_, int Bt_,
+                                          const double *Vb_, double *Yb_) {
+    extern __shared__ double lds[];
+    const gdouble *const gA = (const gdouble *)uni_ptr(A_);
+    const double *const Vb = uni_ptr(Vb_);
+    double *const Yb = uni_ptr(Yb_);
+    const int n = uni(n_), npad = uni(npad_), nt = uni(nt_), C = uni(C_), Bt = uni(Bt_);
+    const int tid = threadIdx.x, w = uni(tid >> 6), lane = tid & 63;
+    const int bi = w & 3, bj = (w >> 2) & 1, h = w >> 3;
+    const int cnt = nt - Bt;
+    double ra[4][4], rv[4][2];
+    blk_d4 acc = {0.0, 0.0, 0.0, 0.0};
+    // loads outside the matrix read a clamped address and are zeroed by a
+    // factor (a guarded select would become a branch of its own with a wait,
+    // which serialises the loads and blurs the wait counts of the ring)
+#define SY_LOAD(S, R)                                                                           \
+    if ((R) < nt) {                                                                             \
+        _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                         \
+            const int e = tid + TRI_NT * u, rw = 64 * C + (e >> 6), cl = 64 * (R) + (e & 63);    \
+            const double f = (rw < n && cl < n) ? 1.0 : 0.0;                                    \
+            ra[S][u] = f * gA[(size_t)min(rw, n - 1) * n + min(cl, n - 1)];                     \
+        }                                                                                       \
+        _Pragma("unroll") for (int u = 0; u < 2; ++u) {                                         \
+            const int e = tid + TRI_NT * u;                                                     \
+            rv[S][u] = ld_dev(Vb + (size_t)(e >> 6) * npad + 64 * (R) + (e & 63));               \
+        }                                                                                       \
+    }
+#define SY_PUT(S, BUF)                                                                          \
+    {                                                                                           \
+        _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                         \
+            const int e = tid + TRI_NT * u;                                                     \
+            lds[SB_LA + (BUF) * 64 * 65 + (e >> 6) * 65 + (e & 63)] = ra[S][u];                 \
+        }                                                                                       \
+        _Pragma("unroll") for (int u = 0; u < 2; ++u) {                                         \
+            const int e = tid + TRI_NT * u;                                                     \
+            lds[SB_LV + (BUF) * 32 * 65 + (e >> 6) * 65 + (e & 63)] = rv[S][u];                 \
+        }                                                                                       \
+    }
+#define SY_STEP(J)                                                                              \
+    if (i0_ + (J) < cnt) {                                                                      \
+        const int R = Bt + i0_ + (J), buf = (i0_ + (J)) & 1;                                    \
+        if (R + 1 < nt) SY_PUT(((J) + 1) & 3, buf ^ 1)                                          \
+        SY_LOAD(((J) + 3) & 3, R + 3)                                                           \
+        _Pragma("unroll") for (int s = 8 * h; s < 8 * h + 8; ++s) {                             \
+            const double a = lds[SB_LA + buf * 64 * 65 + (16 * bi + (lane & 15)) * 65 + 4 * s + (lane >> 4)]; \
+            const double b = lds[SB_LV + buf * 32 * 65 + (16 * bj + (lane & 15)) * 65 + 4 * s + (lane >> 4)]; \
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);                    \
+        }                                                                                       \
+        __syncthreads();                                                                        \
+    }
+    SY_LOAD(0, Bt)
+    SY_LOAD(1, Bt + 1)
+    SY_LOAD(2, Bt + 2)
+    SY_PUT(0, 0)
+    __syncthreads();
+    SB_RING4(SY_STEP)
+#undef SY_LOAD
+#undef SY_PUT
+#undef SY_STEP
+    if (h == 1) {
 #pragma unroll
-    for (int q = 0; q < 2 * SBR_B * 64 / TRI_NT; ++q) {
-        const int e = tid + q * TRI_NT, k = e >> 6, i = e & 63;
-        const int kk = k < SBR_B ? k : k - SBR_B;
-        const int c = 64 * R + i;
-        Rs[e] = k < SBR_B ? ld_dev(Wb + (size_t)kk * npad + c) : ld_dev(Vb + (size_t)kk * npad + c);
+        for (int v = 0; v < 4; ++v) lds[SB_RED + (w & 7) * 256 + v * 64 + lane] = acc[v];
     }
     __syncthreads();
+    if (h == 0) {
 #pragma unroll
-    for (int s = 0; s < 2 * SBR_B / 4; ++s) {
-        const int k = 4 * s + (lane >> 4);
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ls[k * 64 + 16 * bi + (lane & 15)], Rs[k * 64 + 16 * bj + (lane & 15)],
-                                                   acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int r = row0 + 4 * v;
-        if (r >= c0 && r < n && col >= c0 && col < n) Ag[(size_t)r * n + col] = acc[v];
+        for (int v = 0; v < 4; ++v) {
+            const double y = acc[v] + lds[SB_RED + (w & 7) * 256 + v * 64 + lane];
+            const int rr = 64 * C + 16 * bi + 4 * v + (lane >> 4), kk = 16 * bj + (lane & 15);
+            Yb[(size_t)rr * SBR_B + kk] = y;
+        }
     }
     __syncthreads();
 }
+
+// A[r][c] -= sum_k V[r][k] W[c][k] + W[r][k] V[c][k] on the trailing tile
+// row of block C, rows and columns >= c0 (both triangles).  The row
+// operands -[V W] of block C are staged once (k-major); per tile R the
+// column operands [W V] go through a double LDS buffer and the old tile is
+// the MFMA accumulator of wavefront w's 16 x 16 block (w / 4, w % 4).
+__device__ __noinline__ void sbr_syr2k_row(double *A_, int n_, int npad_, int nt_, int C_, int Bt_, int c0_,
+                                           const double *Vb_, const double *Wb_) {
+    extern __shared__ double lds[];
+    gdouble *const gA = (gdouble *)uni_ptr(A_);
+    const double *const Vb = uni_ptr(Vb_), *const Wb = uni_ptr(Wb_);
+    const int n = uni(n_), npad = uni(npad_), nt = uni(nt_), C = uni(C_), Bt = uni(Bt_), c0 = uni(c0_);
+    const int tid = threadIdx.x, w = uni(tid >> 6), lane = tid & 63;
+    const int bi = w >> 2, bj = w & 3;
+    const int row0 = 64 * C + 16 * bi + (lane >> 4);  // element v: row row0 + 4 v
+    const int cnt = nt - Bt;
+    double *const Ls = lds + SB_LS;
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 2 * SBR_B * 64 / TRI_NT; ++u) {
+        const int e = tid + TRI_NT * u, k = e >> 6, i = e & 63;
+        const int kk = k < SBR_B ? k : k - SBR_B;
+        const int rw = 64 * C + i;
+        Ls[e] = -(k < SBR_B ? ld_dev(Vb + (size_t)kk * npad + rw) : ld_dev(Wb + (size_t)kk * npad + rw));
+    }
+    double ro[4][4], rr[4][4];
+#define SR_LOAD(S, R)                                                                           \
+    if ((R) < nt) {                                                                             \
+        const int col = 64 * (R) + 16 * bj + (lane & 15);                                       \
+        _Pragma("unroll") for (int v = 0; v < 4; ++v) {                                         \
+            const double f = (row0 + 4 * v < n && col < n) ? 1.0 : 0.0;                         \
+            ro[S][v] = f * gA[(size_t)min(row0 + 4 * v, n - 1) * n + min(col, n - 1)];          \
+        }                                                                                       \
+        _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                         \
+            const int e = tid + TRI_NT * u, k = e >> 6, i = e & 63;                             \
+            const int kk = k < SBR_B ? k : k - SBR_B;                                           \
+            const int c = 64 * (R) + i;                                                         \
+            rr[S][u] = k < SBR_B ? ld_dev(Wb + (size_t)kk * npad + c) : ld_dev(Vb + (size_t)kk * npad + c); \
+        }                                                                                       \
+    }
+#define SR_PUT(S, BUF)                                                                          \
+    {                                                                                           \
+        _Pragma("unroll") for (int u = 0; u < 4; ++u) lds[SB_RS + (BUF) * 64 * 64 + tid + TRI_NT * u] = rr[S][u]; \
+    }
+#define SR_STEP(J)                                                                              \
+    if (i0_ + (J) < cnt) {                                                                      \
+        const int R = Bt + i0_ + (J), buf = (i0_ + (J)) & 1;                                    \
+        if (R + 1 < nt) SR_PUT(((J) + 1) & 3, buf ^ 1)                                          \
+        SR_LOAD(((J) + 3) & 3, R + 3)                                                           \
+        blk_d4 acc;                                                                             \
+        _Pragma("unroll") for (int v = 0; v < 4; ++v) acc[v] = ro[(J) & 3][v];                  \
+        const double *Rs = lds + SB_RS + buf * 64 * 64;                                         \
+        _Pragma("unroll") for (int s = 0; s < 2 * SBR_B / 4; ++s) {                             \
+            const int k = 4 * s + (lane >> 4);                                                  \
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Ls[k * 64 + 16 * bi + (lane & 15)],     \
+                                                       Rs[k * 64 + 16 * bj + (lane & 15)], acc, 0, 0, 0); \
+        }                                                                                       \
+        const int col = 64 * R + 16 * bj + (lane & 15);                                         \
+        _Pragma("unroll") for (int v = 0; v < 4; ++v) {                                         \
+            const int r = row0 + 4 * v;                                                         \
+            if (r >= c0 && r < n && col >= c0 && col < n) gA[(size_t)r * n + col] = acc[v];    \
+        }                                                                                       \
+        __syncthreads();                                                                        \
+    }
+    SR_LOAD(0, Bt)
+    SR_LOAD(1, Bt + 1)
+    SR_LOAD(2, Bt + 2)
+    SR_PUT(0, 0)
+    __syncthreads();
+    SB_RING4(SR_STEP)
+#undef SR_LOAD
+#undef SR_PUT
+#undef SR_STEP
+}
+#undef SB_RING4
 
 // the number of owned blocks in [B0, nt), and the q-th of them (-1: none)
 __device__ __forceinline__ int sbr_nowned(int B0, int nt, int rank, int T) {
@@ -1644,65 +1768,10 @@ __device__ __noinline__ void sbr_blk(const EigMat &mt, int rank, int T, int *bar
         ++phase;
         if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
         SB_PROF(4);
-        // ---- Y = A22 V on the owned tile rows: wavefront w computes the 16 x 16
-        // block (w & 3, (w >> 2) & 1) of Y[C] over half h = w >> 3 of each
-        // tile's 64 columns; the halves are added in order at the end
+        // ---- Y = A22 V on the owned tile rows, then per-block Gram partials
         for (int q = 0; q < nq; ++q) {
             const int C = sbr_qblock(Bt, nt, rank, T, q);
-            const int bi = w & 3, bj = (w >> 2) & 1, h = w >> 3;
-            blk_d4 acc = {0.0, 0.0, 0.0, 0.0};
-            double ta[4], tv[2];
-            auto fetch = [&](int R) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int e = tid + TRI_NT * u, rr = e >> 6, cc = e & 63;
-                    const int rw = 64 * C + rr, cl = 64 * R + cc;
-                    const double x = gA[(size_t)min(rw, n - 1) * n + min(cl, n - 1)];
-                    ta[u] = (rw < n && cl < n) ? x : 0.0;
-                }
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const int e = tid + TRI_NT * u, kk = e >> 6, cc = e & 63;
-                    tv[u] = ld_dev(Vb + (size_t)kk * npad + 64 * R + cc);
-                }
-            };
-            fetch(Bt);
-            for (int R = Bt; R < nt; ++R) {
-                __syncthreads();
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int e = tid + TRI_NT * u;
-                    lds[SB_LA + (e >> 6) * 65 + (e & 63)] = ta[u];
-                }
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    const int e = tid + TRI_NT * u;
-                    lds[SB_LV + (e >> 6) * 65 + (e & 63)] = tv[u];
-                }
-                __syncthreads();
-                if (R + 1 < nt) fetch(R + 1);
-#pragma unroll
-                for (int s = 8 * h; s < 8 * h + 8; ++s) {
-                    const double a = lds[SB_LA + (16 * bi + (lane & 15)) * 65 + 4 * s + (lane >> 4)];
-                    const double b = lds[SB_LV + (16 * bj + (lane & 15)) * 65 + 4 * s + (lane >> 4)];
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-                }
-            }
-            __syncthreads();
-            if (h == 1) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) lds[SB_RED + (w & 7) * 256 + v * 64 + lane] = acc[v];
-            }
-            __syncthreads();
-            if (h == 0) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const double y = acc[v] + lds[SB_RED + (w & 7) * 256 + v * 64 + lane];
-                    const int rr = 64 * C + 16 * bi + 4 * v + (lane >> 4), kk = 16 * bj + (lane & 15);
-                    Yb[(size_t)rr * SBR_B + kk] = y;
-                }
-            }
-            __syncthreads();
+            sbr_symm_row(A, n, npad, nt, C, Bt, Vb, Yb);
             // Gram partials of block C: G = V^T V, M = V^T Y over its rows
             {
                 double *const Lv = lds + SB_LA;         // V[C] as [row][k] (64 x 33)
@@ -1826,19 +1895,7 @@ __device__ __noinline__ void sbr_blk(const EigMat &mt, int rank, int T, int *bar
         if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
         SB_PROF(9);
         // ---- A22 -= V W^T + W V^T on the owned tile rows
-        for (int q = 0; q < nq; ++q) {
-            const int C = sbr_qblock(Bt, nt, rank, T, q);
-            __syncthreads();
-#pragma unroll
-            for (int u = 0; u < 2 * SBR_B * 64 / TRI_NT; ++u) {
-                const int e = tid + TRI_NT * u, k = e >> 6, i = e & 63;
-                const int kk = k < SBR_B ? k : k - SBR_B;
-                const int rw = 64 * C + i;
-                lds[SB_LS + e] = -(k < SBR_B ? ld_dev(Vb + (size_t)kk * npad + rw) : ld_dev(Wb + (size_t)kk * npad + rw));
-            }
-            __syncthreads();
-            for (int R = Bt; R < nt; ++R) sbr_syr2k_tile(A, n, npad, C, R, c0, Vb, Wb);
-        }
+        for (int q = 0; q < nq; ++q) sbr_syr2k_row(A, n, npad, nt, sbr_qblock(Bt, nt, rank, T, q), Bt, c0, Vb, Wb);
         SB_PROF(10);
     }
 #ifdef NRG_SBR_PROF
@@ -1985,10 +2042,22 @@ __device__ __forceinline__ bool ch_sweep(const EigMat &mt, int s, double (*ring)
     double *const gdg = vslot(mt.ws, n, V_DG), *const god = vslot(mt.ws, n, V_OD);
     auto slot = [&](int j) { return (j - s - 1 + CH_W) % CH_W; };
     auto L = [&](int j, int d) -> double & { return ring[slot(j)][d]; };
+    // (at most b + 1 columns: the loads of a thread are issued together,
+    // then stored to LDS; a load and its store in one loop iteration would
+    // cost one round trip per entry)
+    constexpr int LPT = (SBR_B + 1) * CH_W / CH_NT + 1;
     auto load_cols = [&](int ja, int jb) {
-        for (int e = tid; e < (jb - ja) * CH_W; e += CH_NT) {
-            const int j = ja + e / CH_W, d = e % CH_W;
-            ring[slot(j)][d] = ld_dev(band + (size_t)j * CH_W + d);
+        const int ne = (jb - ja) * CH_W;
+        double r[LPT];
+#pragma unroll
+        for (int u = 0; u < LPT; ++u) {
+            const int e = min(tid + CH_NT * u, ne - 1);
+            r[u] = ld_dev(band + (size_t)(ja + e / CH_W) * CH_W + e % CH_W);
+        }
+#pragma unroll
+        for (int u = 0; u < LPT; ++u) {
+            const int e = tid + CH_NT * u;
+            if (e < ne) ring[slot(ja + e / CH_W)][e % CH_W] = r[u];
         }
     };
     auto store_cols = [&](int ja, int jb) {
@@ -2144,17 +2213,18 @@ __global__ __launch_bounds__(CH_NT) void k_sbr_chase(const EigMat *mats, const S
 //     vectors are streamed through once per block.
 // ---------------------------------------------------------------------------
 // One wavefront per 64 vectors (lane = vector), BL sweeps per block; the
-// next step's new window entries are loaded while this step computes.
-// (Measured on the keep-10000 block set: BL = 48 1.14 s, 32 1.30 s, 64
-// with spills 1.60 s; LDS-transposed coalesced window traffic 1.59 s and
-// four wavefronts sharing a staging 1.68 s: the reflectors' LDS broadcast
-// reads, not HBM, bound the step.)
+// next step's new window entries are loaded while this step computes, its
+// reflectors after it (into the other of two LDS buffers).  Measured on the
+// keep-10000 block set: BL = 48 1.14 s, 32 1.30 s, 64 (spills) 1.60 s; the
+// window traffic coalesced through an LDS transpose 1.59 s, four wavefronts
+// sharing one staging 1.68 s, reflectors a step ahead in registers 1.70 s
+// (occupancy): the reflectors' LDS broadcast reads bound a step, not HBM.
 constexpr int SBT_BL = 48;  // sweeps per block of the stage-2 back-transformation
 template <int BL>
 __global__ __launch_bounds__(64) void k_sbr_bt(const EigMat *mats, const BtItem *items) {
     constexpr int WIN = SBR_B + BL - 1;  // window length
-    __shared__ double rv[BL][SBR_B];
-    __shared__ double rt[BL];
+    __shared__ double rv[2][BL][SBR_B];
+    __shared__ double rt[2][BL];
     const BtItem it = items[blockIdx.x];
     const EigMat mt = mats[it.mat];
     const int n = mt.n, lane = threadIdx.x;
@@ -2165,48 +2235,70 @@ __global__ __launch_bounds__(64) void k_sbr_bt(const EigMat *mats, const BtItem 
     const int nsw = n - 1;  // sweeps 0 .. n - 2
     const int nblk = (nsw + BL - 1) / BL;
     double z[WIN], zn[SBR_B];
+    // reflectors H(s, k), s in [s0, s1) (row n-1-s, columns k b ..; tau in
+    // place of the unit first element) into LDS buffer buf
+    // (loads issued in batches of RB before their LDS stores: a load and
+    // its store in one loop iteration wait for each other, one round trip
+    // per entry)
+    constexpr int RPL = BL * SBR_B / 64, RB = RPL / 2;
+    static_assert(RPL % RB == 0, "reflector staging batches");
+    auto stage_refl = [&](int s0, int s1, int k, int buf) {
+        const int base = s0 + 1 + k * SBR_B;
+#pragma unroll
+        for (int u0 = 0; u0 < RPL; u0 += RB) {
+            double r[RB];
+#pragma unroll
+            for (int u = 0; u < RB; ++u) {
+                const int e = lane + 64 * (u0 + u), o = e / SBR_B, t = e % SBR_B, s = s0 + o;
+                const int st = base + o;  // first index of H(s, k)
+                const bool ok = s < s1 && st < n && st + t < n;
+                const double f = ok ? 1.0 : 0.0;
+                r[u] = f * A[ok ? (size_t)(n - 1 - s) * n + k * SBR_B + t : 0];
+            }
+#pragma unroll
+            for (int u = 0; u < RB; ++u) {
+                const int e = lane + 64 * (u0 + u), o = e / SBR_B, t = e % SBR_B;
+                if (t == 0) {
+                    rt[buf][o] = r[u];
+                    rv[buf][o][0] = 1.0;
+                } else {
+                    rv[buf][o][t] = r[u];
+                }
+            }
+        }
+    };
     for (int bi = nblk - 1; bi >= 0; --bi) {
         const int s0 = bi * BL, s1 = min(s0 + BL, nsw);
         int base = s0 + 1;
 #pragma unroll
         for (int p = 0; p < WIN; ++p) z[p] = (act && base + p < n) ? zrow[base + p] : 0.0;
-        for (int k = 0; base < n; ++k, base += SBR_B) {
+        stage_refl(s0, s1, 0, 0);
+        wave_lds_sync();
+        for (int k = 0, buf = 0; base < n; ++k, base += SBR_B, buf ^= 1) {
             const bool more = base + SBR_B < n;
 #pragma unroll
             for (int p = 0; p < SBR_B; ++p) {
                 const int idx = base + WIN + p;
                 zn[p] = (act && more && idx < n) ? zrow[idx] : 0.0;
             }
-            // reflectors H(s, k), s in [s0, s1): row n-1-s, columns k b ..
-            for (int e = lane; e < BL * SBR_B; e += 64) {
-                const int o = e / SBR_B, t = e % SBR_B, s = s0 + o;
-                const int st = base + o;  // first index of H(s, k)
-                double v = 0.0;
-                if (s < s1 && st < n && st + t < n) v = A[(size_t)(n - 1 - s) * n + k * SBR_B + t];
-                if (t == 0) {
-                    rt[o] = (s < s1 && st < n) ? v : 0.0;
-                    v = 1.0;
-                }
-                rv[o][t] = v;
-            }
-            wave_lds_sync();
 #pragma unroll
             for (int o = BL - 1; o >= 0; --o) {
-                const double tau = rt[o];
+                const double tau = rt[buf][o];
                 if (tau == 0.0) continue;
                 // four partial sums: a chain of 8 dependent FMAs instead of 32
                 double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
 #pragma unroll
                 for (int t = 0; t < SBR_B; t += 4) {
-                    d0 = fma(rv[o][t], z[o + t], d0);
-                    d1 = fma(rv[o][t + 1], z[o + t + 1], d1);
-                    d2 = fma(rv[o][t + 2], z[o + t + 2], d2);
-                    d3 = fma(rv[o][t + 3], z[o + t + 3], d3);
+                    d0 = fma(rv[buf][o][t], z[o + t], d0);
+                    d1 = fma(rv[buf][o][t + 1], z[o + t + 1], d1);
+                    d2 = fma(rv[buf][o][t + 2], z[o + t + 2], d2);
+                    d3 = fma(rv[buf][o][t + 3], z[o + t + 3], d3);
                 }
                 const double d = tau * ((d0 + d1) + (d2 + d3));
 #pragma unroll
-                for (int t = 0; t < SBR_B; ++t) z[o + t] = fma(-d, rv[o][t], z[o + t]);
+                for (int t = 0; t < SBR_B; ++t) z[o + t] = fma(-d, rv[buf][o][t], z[o + t]);
             }
+            if (more) stage_refl(s0, s1, k + 1, buf ^ 1);
             wave_lds_sync();
             // entries base .. base + b - 1 are final for this block
 #pragma unroll
