@@ -1417,7 +1417,7 @@ constexpr int SB_M = SB_G + 32 * 33;
 constexpr int SB_CO = SB_M + 32 * 33;         // per-reflector coefficients, taus
 constexpr int SB_PS = SB_CO + 128;            // partial-sum chunks (32 x 33)
 constexpr int SB_TOT = SB_PS + 32 * 33;          // their totals
-constexpr int SB_END = SB_TOT + 32;
+constexpr int SB_END = SB_TOT + 64;  // (totals, then the pivot row)
 constexpr int SB_LS = 0, SB_RS = 64 * 64;     // rank-2k operands: rows (64 x 64), columns (2 x 64 x 64), k-major
 static_assert(SB_RS + 2 * 64 * 64 <= SB_T, "rank-2k operands overlap only the product's buffers");
 static_assert(sizeof(double) * SB_END <= TRI_COOP_LDS, "stage 1 fits the team launch's LDS");
@@ -1599,31 +1599,48 @@ __device__ __forceinline__ int sbr_qblock(int B0, int nt, int rank, int T, int q
     return -1;
 }
 
-__device__ __noinline__ void sbr_blk(const EigMat &mt, int rank, int T, int *bar, int *status, int *s_flag) {
-    extern __shared__ double lds[];
-    const int n = mt.n, npad = (n + 63) & ~63, nt = npad >> 6;
-    double *__restrict__ A = mt.A;
-    double *ws = mt.ws;
-    double *const base = (double *)(((uintptr_t)(ws + (size_t)n * n) + 511) & ~(uintptr_t)511);
-    double *const Vb = base;
-    double *const Wb = Vb + (size_t)SBR_B * npad;
-    double *const Yb = Wb + (size_t)SBR_B * npad;
-    double *const part = Yb + (size_t)SBR_B * npad;  // [parity][block][SBR_B]
-    double *const piv = part + 2 * (size_t)nt * SBR_B;  // [parity][SBR_B]
-    double *const gram = piv + 2 * SBR_B;             // [block][2][SBR_B][SBR_B]
-    double *const gtot = gram + (size_t)nt * 2 * SBR_B * SBR_B;  // [2][SBR_B][SBR_B]
-    double *const tau_g = vslot(ws, n, V_TAU);
-    gdouble *const gA = (gdouble *)A;
-    const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
-    double *const co = lds + SB_CO;  // [0..31] s_i, [32] tau, [33] beta, [34] scale, [64..95] taus
-    int phase = 0;
+// the sum over blocks B = Bs + g + 32 u < nt of p[B * SBR_B] (NU loads issued together)
+template <int NU>
+__device__ __forceinline__ double sbr_bsum(const double *p, int Bs, int g, int nt) {
+    double t[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) t[u] = ld_dev(p + (size_t)min(Bs + g + 32 * u, nt - 1) * SBR_B);
+    double s = 0.0;
+#pragma unroll
+    for (int u = 0; u < NU; ++u) s += Bs + g + 32 * u < nt ? t[u] : 0.0;
+    return s;
+}
+
+// Per-job constants of a stage-1 team member, kept in LDS (the phases are
+// separate functions, each with registers of its own: as one function the
+// phases spilled heavily).
+struct SbrCtx {
+    double *A, *ws, *Vb, *Wb, *Yb, *part, *piv, *gram, *gtot, *tau_g;
+    int *bar, *status, *s_flag;
+    int n, npad, nt, rank, T;
 #ifdef NRG_SBR_PROF
-    long long pacc[12] = {0}, plast = wall_clock64();
-#define SB_PROF(i) do { __syncthreads(); if (tid == 0) { const long long now_ = wall_clock64(); pacc[i] += now_ - plast; plast = now_; } } while (0)
+    long long pacc[12], plast;
+#endif
+};
+#ifdef NRG_SBR_PROF
+#define SB_PROF(i) do { __syncthreads(); if (tid == 0) { const long long now_ = wall_clock64(); sc->pacc[i] += now_ - sc->plast; sc->plast = now_; } } while (0)
 #else
 #define SB_PROF(i) do {} while (0)
 #endif
 
+__device__ __noinline__ void sbr_mirror(const SbrCtx *sc) {
+    extern __shared__ double lds[];
+    const int n = sc->n, npad = sc->npad, nt = sc->nt, rank = sc->rank, T = sc->T;
+    double *__restrict__ A = sc->A;
+    double *ws = sc->ws;
+    double *const Vb = sc->Vb, *const Wb = sc->Wb, *const Yb = sc->Yb, *const part = sc->part, *const piv = sc->piv;
+    double *const gram = sc->gram, *const gtot = sc->gtot, *const tau_g = sc->tau_g;
+    int *const bar = sc->bar, *const status = sc->status, *const s_flag = sc->s_flag;
+    gdouble *const gA = (gdouble *)A;
+    const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    double *const co = lds + SB_CO;  // [0..31] s_i, [32] tau, [33] beta, [34] scale, [64..95] taus
+    (void)ws; (void)Vb; (void)Wb; (void)Yb; (void)part; (void)piv; (void)gram; (void)gtot; (void)tau_g;
+    (void)bar; (void)status; (void)s_flag; (void)gA; (void)w; (void)lane; (void)co; (void)npad; (void)rank;
     // 0. mirror the upper triangle into the lower one on the owned rows:
     //    tile (C', B) (another rank's rows, written before this launch) is
     //    read whole and written transposed into tile (B, C')
@@ -1648,10 +1665,24 @@ __device__ __noinline__ void sbr_blk(const EigMat &mt, int rank, int T, int *bar
         }
     }
 
-    SB_PROF(0);
-    for (int j0 = 0;; j0 += SBR_B) {
+}
+
+// panel j0: QR of its lower copy (one barrier per reflector), the write-back
+// of R, the reflectors and V; returns the team phase, -1 if a barrier failed
+__device__ __noinline__ int sbr_qr(SbrCtx *sc, int j0, int phase) {
+    extern __shared__ double lds[];
+    const int n = sc->n, npad = sc->npad, nt = sc->nt, rank = sc->rank, T = sc->T;
+    double *__restrict__ A = sc->A;
+    double *ws = sc->ws;
+    double *const Vb = sc->Vb, *const Wb = sc->Wb, *const Yb = sc->Yb, *const part = sc->part, *const piv = sc->piv;
+    double *const gram = sc->gram, *const gtot = sc->gtot, *const tau_g = sc->tau_g;
+    int *const bar = sc->bar, *const status = sc->status, *const s_flag = sc->s_flag;
+    gdouble *const gA = (gdouble *)A;
+    const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    double *const co = lds + SB_CO;  // [0..31] s_i, [32] tau, [33] beta, [34] scale, [64..95] taus
+    (void)ws; (void)Vb; (void)Wb; (void)Yb; (void)part; (void)piv; (void)gram; (void)gtot; (void)tau_g;
+    (void)bar; (void)status; (void)s_flag; (void)gA; (void)w; (void)lane; (void)co; (void)npad; (void)rank;
         const int c0 = j0 + SBR_B, m = n - c0;
-        if (m < 2) break;
         const int Bt = c0 >> 6;
         const int kmax = min(SBR_B, m - 1);
         const int nq = min(sbr_nowned(Bt, nt, rank, T), SBR_MAXQ);
@@ -1687,37 +1718,31 @@ __device__ __noinline__ void sbr_blk(const EigMat &mt, int rank, int T, int *bar
             }
             SB_PROF(1);
             ++phase;
-            if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
+            if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return -1;
             SB_PROF(2);
             // block sums in block order: value i (0: |x|^2) by lanes i, blocks
-            // B = Bs + g + 32 u by thread group g (loads issued together), the
-            // 32 group sums added in order
+            // B = Bs + g + 32 u by thread group g, the 32 group sums added in
+            // order; the pivot row's values come in the same round trip
             {
-                const int Bs = (rp + 1) >> 6;
+                const int Bs = (rp + 1) >> 6, nbk = nt - Bs;
                 const int i = tid & 31, g = tid >> 5;
+                const double pv = g == 0 ? ld_dev(piv + par * SBR_B + i) : 0.0;
                 double s = 0.0;
                 if (i == 0 || i > k) {
-                    double t[16];
-#pragma unroll
-                    for (int u = 0; u < 16; ++u) {
-                        const int B = min(Bs + g + 32 * u, nt - 1);
-                        t[u] = ld_dev(part + ((size_t)par * nt + B) * SBR_B + i);
-                    }
-#pragma unroll
-                    for (int u = 0; u < 16; ++u) s += Bs + g + 32 * u < nt ? t[u] : 0.0;
+                    const double *pp = part + (size_t)par * nt * SBR_B + i;
+                    if (nbk <= 4 * 32) s = sbr_bsum<4>(pp, Bs, g, nt);
+                    else if (nbk <= 8 * 32) s = sbr_bsum<8>(pp, Bs, g, nt);
+                    else s = sbr_bsum<16>(pp, Bs, g, nt);
                 }
                 lds[SB_PS + g * 33 + i] = s;
-                __syncthreads();
-                if (tid < 32) {
-                    double t = 0.0;
-                    for (int gg = 0; gg < 32; ++gg) t += lds[SB_PS + gg * 33 + tid];
-                    lds[SB_TOT + tid] = t;
-                }
+                if (g == 0) lds[SB_TOT + 32 + i] = pv;
                 __syncthreads();
             }
             if (tid < 32) {
-                const double sigma = lds[SB_TOT];
-                const double alpha = ld_dev(piv + par * SBR_B + k);
+                double tot = 0.0;
+                for (int gg = 0; gg < 32; ++gg) tot += lds[SB_PS + gg * 33 + tid];
+                const double sigma = readlane_d(tot, 0);
+                const double alpha = lds[SB_TOT + 32 + k];
                 double tau = 0.0, beta = alpha, scale = 0.0;
                 if (sigma != 0.0) {
                     beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
@@ -1726,7 +1751,7 @@ __device__ __noinline__ void sbr_blk(const EigMat &mt, int rank, int T, int *bar
                 }
                 const int i = tid;
                 double si = 0.0;
-                if (i > k) si = tau * (ld_dev(piv + par * SBR_B + i) + lds[SB_TOT + i] * scale);
+                if (i > k) si = tau * (lds[SB_TOT + 32 + i] + tot * scale);
                 co[i] = si;
                 if (i == 0) {
                     co[32] = tau;
@@ -1746,7 +1771,8 @@ __device__ __noinline__ void sbr_blk(const EigMat &mt, int rank, int T, int *bar
                     else if (i > k && (r >= rp) && rok) p[i] -= co[i] * v;
                 }
             }
-            __syncthreads();  // co is rewritten by the next reflector
+            // (co and the partial-sum table are next written after the next
+            // team barrier, which every wavefront reaches after this update)
             SB_PROF(3);
         }
         for (int k = kmax + tid; k < SBR_B; k += TRI_NT) {
@@ -1766,8 +1792,29 @@ __device__ __noinline__ void sbr_blk(const EigMat &mt, int rank, int T, int *bar
             }
         }
         ++phase;
-        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
+        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return -1;
         SB_PROF(4);
+    return phase;
+}
+
+// panel j0: Y = A22 V, the Gram matrices, T, S, W, and the rank-64 update of
+// the owned tile rows; returns the team phase, -1 if a barrier failed
+__device__ __noinline__ int sbr_upd(SbrCtx *sc, int j0, int phase) {
+    extern __shared__ double lds[];
+    const int n = sc->n, npad = sc->npad, nt = sc->nt, rank = sc->rank, T = sc->T;
+    double *__restrict__ A = sc->A;
+    double *ws = sc->ws;
+    double *const Vb = sc->Vb, *const Wb = sc->Wb, *const Yb = sc->Yb, *const part = sc->part, *const piv = sc->piv;
+    double *const gram = sc->gram, *const gtot = sc->gtot, *const tau_g = sc->tau_g;
+    int *const bar = sc->bar, *const status = sc->status, *const s_flag = sc->s_flag;
+    gdouble *const gA = (gdouble *)A;
+    const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    double *const co = lds + SB_CO;  // [0..31] s_i, [32] tau, [33] beta, [34] scale, [64..95] taus
+    (void)ws; (void)Vb; (void)Wb; (void)Yb; (void)part; (void)piv; (void)gram; (void)gtot; (void)tau_g;
+    (void)bar; (void)status; (void)s_flag; (void)gA; (void)w; (void)lane; (void)co; (void)npad; (void)rank;
+        const int c0 = j0 + SBR_B;
+        const int Bt = c0 >> 6;
+        const int nq = min(sbr_nowned(Bt, nt, rank, T), SBR_MAXQ);
         // ---- Y = A22 V on the owned tile rows, then per-block Gram partials
         for (int q = 0; q < nq; ++q) {
             const int C = sbr_qblock(Bt, nt, rank, T, q);
@@ -1798,7 +1845,7 @@ __device__ __noinline__ void sbr_blk(const EigMat &mt, int rank, int T, int *bar
         }
         SB_PROF(5);
         ++phase;
-        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
+        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return -1;
         SB_PROF(6);
         // ---- G, M summed over the blocks: element e of the 2 x 32 x 32 by
         // rank e % T (blocks B = Bt + g + 32 u by thread group g, the 32
@@ -1829,26 +1876,34 @@ __device__ __noinline__ void sbr_blk(const EigMat &mt, int rank, int T, int *bar
             }
         }
         ++phase;
-        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
+        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return -1;
         SB_PROF(7);
         // T (dlarft, forward columnwise); S = T^T M T; W = Y T - V S / 2
         {
             const int i = tid >> 5, jj = tid & 31;
             lds[SB_G + i * 33 + jj] = ld_dev(gtot + tid);
             lds[SB_M + i * 33 + jj] = ld_dev(gtot + SBR_B * SBR_B + tid);
-            lds[SB_T + i * 33 + jj] = 0.0;
         }
         __syncthreads();
-        for (int k = 0; k < SBR_B; ++k) {
-            const double tk = co[64 + k];
-            double t = 0.0;
-            if (tid < k)
-                for (int l = tid; l < k; ++l) t = fma(lds[SB_T + tid * 33 + l], lds[SB_G + l * 33 + k], t);
-            __syncthreads();
-            if (tid < k) lds[SB_T + tid * 33 + k] = -tk * t;
-            if (tid == k) lds[SB_T + k * 33 + k] = tk;
-            __syncthreads();
+        __syncthreads();
+        // T in one wavefront: lane i < 32 keeps row i in registers (T is
+        // upper triangular, T[i][k] = -tau_k sum_{l<k} T[i][l] G[l][k])
+        if (w == 0) {
+            double trow[SBR_B];
+#pragma unroll
+            for (int k = 0; k < SBR_B; ++k) {
+                const double tk = co[64 + k];
+                double t = 0.0;
+#pragma unroll
+                for (int l = 0; l < k; ++l) t = fma(trow[l], lds[SB_G + l * 33 + k], t);
+                trow[k] = lane < k ? -tk * t : lane == k ? tk : 0.0;
+            }
+            if (lane < SBR_B) {
+#pragma unroll
+                for (int k = 0; k < SBR_B; ++k) lds[SB_T + lane * 33 + k] = trow[k];
+            }
         }
+        __syncthreads();
         {
             const int i = tid >> 5, jj = tid & 31;
             double t = 0.0;  // (M T)[i][jj]
@@ -1892,19 +1947,67 @@ __device__ __noinline__ void sbr_blk(const EigMat &mt, int rank, int T, int *bar
         }
         SB_PROF(8);
         ++phase;
-        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return;
+        if (!team_barrier(bar, phase * T, status, s_flag, BLK_SPIN_LIMIT)) return -1;
         SB_PROF(9);
         // ---- A22 -= V W^T + W V^T on the owned tile rows
         for (int q = 0; q < nq; ++q) sbr_syr2k_row(A, n, npad, nt, sbr_qblock(Bt, nt, rank, T, q), Bt, c0, Vb, Wb);
         SB_PROF(10);
+    return phase;
+}
+
+__device__ __noinline__ void sbr_blk(const EigMat &mt, int rank, int T, int *bar, int *status, int *s_flag) {
+    __shared__ SbrCtx sc;
+    const int n = mt.n, npad = (n + 63) & ~63, nt = npad >> 6;
+    if (threadIdx.x == 0) {
+        double *const base = (double *)(((uintptr_t)(mt.ws + (size_t)n * n) + 511) & ~(uintptr_t)511);
+        sc.A = mt.A;
+        sc.ws = mt.ws;
+        sc.Vb = base;
+        sc.Wb = sc.Vb + (size_t)SBR_B * npad;
+        sc.Yb = sc.Wb + (size_t)SBR_B * npad;
+        sc.part = sc.Yb + (size_t)SBR_B * npad;          // [parity][block][SBR_B]
+        sc.piv = sc.part + 2 * (size_t)nt * SBR_B;       // [parity][SBR_B]
+        sc.gram = sc.piv + 2 * SBR_B;                    // [block][2][SBR_B][SBR_B]
+        sc.gtot = sc.gram + (size_t)nt * 2 * SBR_B * SBR_B;  // [2][SBR_B][SBR_B]
+        sc.tau_g = vslot(mt.ws, n, V_TAU);
+        sc.bar = bar;
+        sc.status = status;
+        sc.s_flag = s_flag;
+        sc.n = n;
+        sc.npad = npad;
+        sc.nt = nt;
+        sc.rank = rank;
+        sc.T = T;
+#ifdef NRG_SBR_PROF
+        for (int i = 0; i < 12; ++i) sc.pacc[i] = 0;
+        sc.plast = wall_clock64();
+#endif
+    }
+    __syncthreads();
+    const int tid = threadIdx.x;
+    (void)tid;
+    SbrCtx *const scp = &sc;
+    {
+        SbrCtx *sc = scp;
+        (void)sc;
+        sbr_mirror(scp);
+        SB_PROF(0);
+    }
+    int phase = 0;
+    for (int j0 = 0;; j0 += SBR_B) {
+        if (n - (j0 + SBR_B) < 2) break;
+        phase = sbr_qr(scp, j0, phase);
+        if (phase < 0) return;
+        phase = sbr_upd(scp, j0, phase);
+        if (phase < 0) return;
     }
 #ifdef NRG_SBR_PROF
     if (rank == 0 && tid == 0)
         printf("sbrprof n=%d T=%d mirror %lld qr %lld qrbar %lld qrupd %lld wb+bar %lld symm %lld bar %lld gram+bar %lld tsw %lld bar %lld syr2k %lld (x10ns)\n",
-               n, T, pacc[0], pacc[1], pacc[2], pacc[3], pacc[4], pacc[5], pacc[6], pacc[7], pacc[8], pacc[9], pacc[10]);
+               n, T, sc.pacc[0], sc.pacc[1], sc.pacc[2], sc.pacc[3], sc.pacc[4], sc.pacc[5], sc.pacc[6], sc.pacc[7], sc.pacc[8], sc.pacc[9], sc.pacc[10]);
 #endif
-#undef SB_PROF
 }
+#undef SB_PROF
 
 __device__ __forceinline__ void tri_teams(const EigMat *mats, const TriJob *jobs, const int *tbase, const int *qjob,
                                           int *counters, int *status) {
