@@ -3798,22 +3798,25 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     // the queues run them in turn.  NRG_BLK_TDIV > 0 instead gives every
     // matrix nt / NRG_BLK_TDIV ranks.
     {
-        // The two-stage reduction's stage 1 (blkjob 2) pays ~34 barriers per
-        // panel of 32 columns and moves its work into FP64-MFMA tile
-        // products (2 n^3 flops, ~0.15 TF per CU): o(n) ~ 3.5 n us,
-        // W(n) ~ 1.4e-5 n^3 CU-us.
+        // The two-stage reduction's stage 1 (blkjob 2) pays ~35 barriers per
+        // panel of 32 columns (o(n) ~ 3.5 n us) and W(n) ~ 2.7e-5 n^3 CU-us of
+        // tile products (clock probes on MI355X: a 12680-row block with 49
+        // ranks among the keep-10000 blocks 1.07 s, a 13136-row block alone
+        // with 103 ranks 0.65 s).
         auto blk_o1 = [](double n) { return n * std::max(0.0114 * n - 25.0, 10.0); };
         auto blk_w1 = [](double n) { return 0.266 * n * n + 7e-5 * n * n * n; };
         const bool two = c->sbr != 0;
         auto blk_o = [&](double n) { return two ? 3.5 * n : blk_o1(n); };
-        auto blk_w = [&](double n) { return two ? 0.5 * n * n + 1.4e-5 * n * n * n : blk_w1(n); };
+        auto blk_w = [&](double n) { return two ? 0.5 * n * n + 2.7e-5 * n * n * n : blk_w1(n); };
         const int tmax = std::min(TEAM_GLOBAL_MAX, tri_grid);
         // between one rank per four and one per two 64-column blocks: smaller
         // teams measured slower than the model in a full mix (their ranks'
-        // per-block work grows), larger ones were not measured
+        // per-block work grows; two-stage floors of one rank per six blocks:
+        // keep-10000 blocks 1.44 -> 1.76 s), larger ones were not measured
         auto team_for = [&](double n, double M) {
             const int nt = ((int)n + 63) / 64;
-            const int cap = std::max(2, std::min(tmax, nt / 2)), floor_ = std::min(cap, std::max(2, nt / 4));
+            const int cap = std::max(2, std::min(tmax, nt / 2)),
+                      floor_ = std::min(cap, std::max(2, nt / 4));
             const double room = M - blk_o(n);
             if (room <= 0.0) return cap;
             return (int)std::min<double>(cap, std::max<double>(floor_, std::ceil(blk_w(n) / room)));
@@ -3828,13 +3831,26 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                 lo = std::max(lo, blk_o(tasks[i].n));
                 hi = std::max(hi, blk_o(tasks[i].n) + blk_w(tasks[i].n) / 2.0);
             }
-            for (int it = 0; it < 60; ++it) {
-                const double mid = 0.5 * (lo + hi);
-                long sum = 0;
-                for (int i : bj) sum += team_for(tasks[i].n, mid);
-                (sum <= tri_grid ? hi : lo) = mid;
+            long floors = 0;
+            for (int i : bj) floors += team_for(tasks[i].n, 1e300);
+            if (floors <= tri_grid) {
+                for (int it = 0; it < 60; ++it) {
+                    const double mid = 0.5 * (lo + hi);
+                    long sum = 0;
+                    for (int i : bj) sum += team_for(tasks[i].n, mid);
+                    (sum <= tri_grid ? hi : lo) = mid;
+                }
+                M = hi;
+            } else if (two) {
+                // the teams run in turn from the queues: size them for the
+                // average load of the device (total work over the grid), so
+                // the largest matrices are not the critical path
+                double wsum = 0.0;
+                for (int i : bj) wsum += blk_w(tasks[i].n);
+                M = std::max(lo, wsum / tri_grid);
+            } else {
+                M = hi;
             }
-            M = hi;
         }
         for (int i : bj) {
             const double n = tasks[i].n;
