@@ -36,13 +36,18 @@ sys.path.insert(0, PKG)
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and MFMA)
 PEAK_HBM_GBS = 8000.0
 # What bounds each kernel, and which of its counted quantities is algorithmic.
-# k_eig_tri_blk: the blocked tridiagonalisation reads the upper half of the
-# trailing matrix once per column (sum_j 4 (d-j)^2 bytes per block); the
-# streaming one (k_eig_tri_coop) the whole of it; binning / assembly /
-# transfers are byte streams; the contractions run on FP64 MFMA.
+# k_eig_tri_blk: the band reduction of a block of dimension d reads the
+# trailing matrix once for Y = A V and reads and writes it once for the
+# rank-64 update per panel of 32 columns (d^3 / 4 bytes; the one-stage
+# reduction below the two-stage size, sum_j 4 (d-j)^2); the streaming one
+# (k_eig_tri_coop) the whole trailing matrix per column; binning / assembly
+# / transfers are byte streams; the contractions run on FP64 MFMA, the
+# bulge-chasing back-transformation (k_sbr_bt, 2 d^3) on the FP64 FMA units
+# (the same 78.6 TFLOP/s peak); the bulge chasing itself (k_sbr_chase) is
+# bound by the latency of its sequential tasks and is not priced here.
 KERNEL_BOUND = {"k_eig_tri_blk": "hbm", "k_eig_tri_coop": "hbm", "k_spec_bin": "hbm", "k_assemble": "hbm",
                 "k_gather_scatter": "hbm", "k_zero": "hbm", "k_eig_copy": "hbm", "k_gemm": "mfma",
-                "k_gemm_dc": "mfma", "k_gemm_bt": "mfma"}
+                "k_gemm_dc": "mfma", "k_gemm_bt": "mfma", "k_sbr_bt": "mfma"}
 
 CONFIGS = {
     # SIAM QS, keep <= 500 states (Ecut = 8 omega_N), Nmax = 30, DM-NRG spectrum at T=1e-8
