@@ -54,6 +54,12 @@ nrg_status nrg_ctx_create(nrg_ctx **ctx, int device);
 void nrg_ctx_destroy(nrg_ctx *ctx);
 const char *nrg_last_error(const nrg_ctx *ctx);
 nrg_status nrg_sync(nrg_ctx *ctx);
+/* No counterpart in the reference (its LAPACK calls allocate per call):
+ * waits for the stream, then returns the context's eigensolver workspace and
+ * grouped-GEMM partial buffer to the device.  The next call that needs them
+ * allocates them again.  For callers done with eigensolves for a while (the
+ * density-matrix and spectral passes), whose HBM goes to other data. */
+nrg_status nrg_ctx_release_workspace(nrg_ctx *ctx);
 
 nrg_status nrg_dev_alloc(nrg_ctx *ctx, size_t bytes, void **dptr);
 nrg_status nrg_dev_free(nrg_ctx *ctx, void *dptr);

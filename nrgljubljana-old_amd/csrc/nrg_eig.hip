@@ -2301,6 +2301,37 @@ __global__ __launch_bounds__(CH_NT) void k_sbr_chase(const EigMat *mats, const S
     }
 }
 
+// Two-stage matrices: the eigenvectors of the tridiagonal between Vc (row v
+// = vector v) and an entry-major copy Zt in S2 (row p = entry p of every
+// vector), in 64 x 64 tiles through LDS (item: matrix, tile ti * nt + tj of
+// the source).  The stage-2 back-transformation runs on Zt.
+__global__ __launch_bounds__(256) void k_sbr_tr(const EigMat *mats, const BtItem *items, int to_zt) {
+    __shared__ double tile[64][65];
+    const BtItem it = items[blockIdx.x];
+    const EigMat mt = mats[it.mat];
+    const int n = mt.n, nt = (n + 63) / 64;
+    const int ti = it.c0 / nt, tj = it.c0 % nt;
+    double *const Vc = mt.ws, *const Zt = mt.ws + 2 * (size_t)n * n;
+    const double *const src = to_zt ? Vc : Zt;
+    double *const dst = to_zt ? Zt : Vc;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    double r[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int row = 64 * ti + ty + 4 * u, col = 64 * tj + tx;
+        const double v = src[(size_t)min(row, n - 1) * n + min(col, n - 1)];
+        r[u] = (row < n && col < n) ? v : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) tile[ty + 4 * u][tx] = r[u];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+        const int row = 64 * tj + ty + 4 * u, col = 64 * ti + tx;
+        if (row < n && col < n) dst[(size_t)row * n + col] = tile[tx][ty + 4 * u];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // 1g. back-transformation of stage 2: x = Q2 y for every eigenvector y of the
 //     tridiagonal (row of Vc).  Q2 = prod over sweep blocks B = [s0, s0 + L)
@@ -2331,10 +2362,14 @@ __global__ __launch_bounds__(64) void k_sbr_bt(const EigMat *mats, const BtItem 
     const BtItem it = items[blockIdx.x];
     const EigMat mt = mats[it.mat];
     const int n = mt.n, lane = threadIdx.x;
-    const int cv = it.c0 + lane;  // vector (row of Vc)
+    const int cv = it.c0 + lane;  // vector (column of Zt)
     const bool act = cv < n;
-    double *const zrow = mt.ws + (size_t)(act ? cv : 0) * n;
-    const double *const A = mt.A;
+    // the vectors entry-major (Zt row p = entry p of every vector), so that a
+    // window load or store of the 64 lanes is one contiguous 512-byte run
+    gdouble *const Zt = (gdouble *)(mt.ws + 2 * (size_t)n * n);
+    const int cz = act ? cv : 0;
+    auto zt = [&](int p) -> gdouble & { return Zt[(size_t)min(p, n - 1) * n + cz]; };
+    const gdouble *const A = (const gdouble *)mt.A;
     const int nsw = n - 1;  // sweeps 0 .. n - 2
     const int nblk = (nsw + BL - 1) / BL;
     double z[WIN], zn[SBR_B];
@@ -2374,7 +2409,10 @@ __global__ __launch_bounds__(64) void k_sbr_bt(const EigMat *mats, const BtItem 
         const int s0 = bi * BL, s1 = min(s0 + BL, nsw);
         int base = s0 + 1;
 #pragma unroll
-        for (int p = 0; p < WIN; ++p) z[p] = (act && base + p < n) ? zrow[base + p] : 0.0;
+        for (int p = 0; p < WIN; ++p) {
+            const double v = zt(base + p);
+            z[p] = (act && base + p < n) ? v : 0.0;
+        }
         stage_refl(s0, s1, 0, 0);
         wave_lds_sync();
         for (int k = 0, buf = 0; base < n; ++k, base += SBR_B, buf ^= 1) {
@@ -2382,7 +2420,8 @@ __global__ __launch_bounds__(64) void k_sbr_bt(const EigMat *mats, const BtItem 
 #pragma unroll
             for (int p = 0; p < SBR_B; ++p) {
                 const int idx = base + WIN + p;
-                zn[p] = (act && more && idx < n) ? zrow[idx] : 0.0;
+                const double v = zt(idx);
+                zn[p] = (act && more && idx < n) ? v : 0.0;
             }
 #pragma unroll
             for (int o = BL - 1; o >= 0; --o) {
@@ -2406,7 +2445,7 @@ __global__ __launch_bounds__(64) void k_sbr_bt(const EigMat *mats, const BtItem 
             // entries base .. base + b - 1 are final for this block
 #pragma unroll
             for (int p = 0; p < SBR_B; ++p)
-                if (act && base + p < n) zrow[base + p] = z[p];
+                if (act && base + p < n) zt(base + p) = z[p];
 #pragma unroll
             for (int p = 0; p < WIN - SBR_B; ++p) z[p] = z[p + SBR_B];
 #pragma unroll
@@ -3551,7 +3590,7 @@ __global__ __launch_bounds__(BT_NT) void k_eig_bt(const EigMat *mats, const BtIt
 //     applied from the last to the first.  Y^T (zero-padded reflector rows)
 //     lives in S1; G = Y^T Y, T, M and W = Z M^T in S2.
 // ---------------------------------------------------------------------------
-constexpr int WY_NB = 64;
+constexpr int WY_NB = 128;
 constexpr int BT_WY_MIN = 1025;   // register kernel below, WY from here
 constexpr int EIG_MAX_N = 32768;
 
@@ -3582,32 +3621,67 @@ __global__ __launch_bounds__(256) void k_wy_extract(const EigMat *mats, const Bt
 
 // T of the block (dlarft forward / columnwise): T_kk = tau_k,
 // T[0:k, k] = -tau_k T[0:k, 0:k] G[0:k, k], with G = Y^T Y of the block.
-__global__ __launch_bounds__(WY_NB) void k_wy_tfactor(const WyItem *items) {
-    __shared__ double sG[WY_NB][WY_NB + 1];
-    __shared__ double sT[WY_NB][WY_NB + 1];
+// Row i of T needs only its own earlier entries and G, so the four lanes
+// 4 i .. 4 i + 3 of one wavefront form row i alone (lane q sums the terms
+// l = q mod 4, the quad adds its four partial sums; no barriers).  LDS holds
+// the strict upper triangle of G by columns (column k: G[0:k, k] at
+// k (k - 1) / 2) and the rows of T packed (row i: T[i, i:] at
+// i NB - i (i - 1) / 2): 2 x 8 K doubles for NB = 128.
+constexpr int WY_TRI = WY_NB * (WY_NB + 1) / 2;
+constexpr int WY_TF_NT = 4 * WY_NB;
+__global__ __launch_bounds__(WY_TF_NT) void k_wy_tfactor(const WyItem *items) {
+    __shared__ double sG[WY_TRI];
+    __shared__ double sT[WY_TRI];
     const WyItem it = items[blockIdx.x];
-    const int n = it.n, nb = it.nb, i = threadIdx.x;
+    const int n = it.n, nb = it.nb, tid = threadIdx.x, i = tid >> 2, q = tid & 3;
     double *S2 = it.ws + 2 * (size_t)n * n;
     const double *G = S2;
     double *T = S2 + WY_NB * WY_NB;
     const double *tau = vslot(it.ws, n, V_TAU) + it.j0;
-    for (int k = 0; k < nb; ++k) {
-        sG[i][k] = (i < nb) ? G[i * WY_NB + k] : 0.0;
-        sT[i][k] = 0.0;
+    // G's strict upper triangle, column k from row 0: loads issued before
+    // the LDS stores (a load and its store in one iteration serialise)
+    constexpr int GB = (WY_TRI + WY_TF_NT - 1) / WY_TF_NT;
+    {
+        double g[GB];
+#pragma unroll
+        for (int u = 0; u < GB; ++u) {
+            const int e = u * WY_TF_NT + tid;
+            // column k holds entries k (k - 1) / 2 .. k (k + 1) / 2 - 1
+            int k = (int)((1.0 + sqrt(1.0 + 8.0 * (double)e)) * 0.5);
+            while (k * (k - 1) / 2 > e) --k;
+            while ((k + 1) * k / 2 <= e) ++k;
+            const int l = e - k * (k - 1) / 2;
+            const double v = G[(size_t)min(l, WY_NB - 1) * WY_NB + min(k, WY_NB - 1)];
+            g[u] = (e < WY_TRI && k < nb) ? v : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < GB; ++u) {
+            const int e = u * WY_TF_NT + tid;
+            if (e < WY_TRI) sG[e] = g[u];
+        }
     }
+    const int ri = i * WY_NB - i * (i - 1) / 2 - i;  // T[i, l] at ri + l
     __syncthreads();
     for (int k = 0; k < nb; ++k) {
-        const double tk = tau[k];
-        double t = 0.0;
-        if (i < k)
-            for (int l = i; l < k; ++l) t += sT[i][l] * sG[l][k];
-        __syncthreads();
-        if (i < k) sT[i][k] = -tk * t;
-        if (i == k) sT[k][k] = tk;
-        __syncthreads();
+        const double *gk = sG + k * (k - 1) / 2;
+        double t0 = 0.0, t1 = 0.0;
+        int l = q;
+        for (; l + 4 < k; l += 8) {
+            t0 += l >= i ? sT[ri + max(l, i)] * gk[l] : 0.0;
+            t1 += l + 4 >= i ? sT[ri + max(l + 4, i)] * gk[l + 4] : 0.0;
+        }
+        if (l < k) t0 += l >= i ? sT[ri + max(l, i)] * gk[l] : 0.0;
+        double t = t0 + t1;
+        t += dpp_d<0xB1>(t);  // quad_perm [1, 0, 3, 2]
+        t += dpp_d<0x4E>(t);  // quad_perm [2, 3, 0, 1]
+        if (q == 0) {
+            const double tk = tau[k];
+            if (i < k) sT[ri + k] = -tk * t;
+            if (i == k) sT[ri + k] = tk;
+        }
     }
     if (i < nb)
-        for (int k = 0; k < nb; ++k) T[i * WY_NB + k] = sT[i][k];
+        for (int k = q; k < nb; k += 4) T[i * WY_NB + k] = k >= i ? sT[ri + k] : 0.0;
 }
 
 // Row c of Z (the back-transformed eigenvector c) -> row rank(c) of A.
@@ -3761,8 +3835,11 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             cost[i] = 2.0 * n;
             continue;
         }
+        // (S2 also holds the compact-WY scratch of the back-transformation:
+        // G and T of a block, M = T Y^T and W, 2 WY_NB (WY_NB + n) doubles)
         if (tasks[i].n >= blk_min_n && c->sbr && sbr_ws_doubles(tasks[i].n) <= (size_t)tasks[i].n * tasks[i].n &&
-            (size_t)tasks[i].n * (CH_W + 1) + 64 <= (size_t)tasks[i].n * tasks[i].n) {
+            (size_t)tasks[i].n * (CH_W + 1) + 64 <= (size_t)tasks[i].n * tasks[i].n &&
+            2 * (size_t)WY_NB * (WY_NB + tasks[i].n) <= (size_t)tasks[i].n * tasks[i].n) {
             blkjob[i] = 2;  // two-stage; team size below, once all blocked jobs are known
             mats[i].pad = 2;
             continue;
@@ -4117,11 +4194,13 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     std::vector<BtItem> rk, cp, wyrows;  // wyrows: one row per item (extract, permuted copy)
     std::vector<int> wymat;              // matrices back-transformed by WY blocks
     std::vector<BtItem> sbt;             // two-stage: 64 eigenvectors per item (stage 2's reflectors)
+    std::vector<BtItem> str;             // two-stage: 64 x 64 tiles of Vc (entry-major copy and back)
     int wy_blocks = 0;
     for (int i = 0; i < nbig; ++i) {
         const int n = tasks[i].n, ns = (n + 63) / 64;
         if (mats[i].pad == 2) {
             for (int c0 = 0; c0 < n; c0 += 64) sbt.push_back({i, c0});
+            for (int t = 0; t < ns * ns; ++t) str.push_back({i, t});
             wymat.push_back(i);
             wy_blocks = std::max(wy_blocks, (sbr_nrefl(n) + WY_NB - 1) / WY_NB);
             for (int r = 0; r < n; ++r) wyrows.push_back({i, r});
@@ -4216,7 +4295,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                  o_cp = ar.put(cp);
     size_t o_bt[kBtVariants];
     for (int v = 0; v < kBtVariants; ++v) o_bt[v] = ar.put(bt[v]);
-    const size_t o_sbt = ar.put(sbt);
+    const size_t o_sbt = ar.put(sbt), o_str = ar.put(str);
     const size_t o_wyrows = ar.put(wyrows), o_wyitems = ar.put(wyitems), o_wyouts = ar.put(wyouts),
                  o_wyterms = ar.put(wyterms), o_wygi = ar.put(wygi), o_wyred = ar.put(wyred);
     if (gslots) {
@@ -4355,8 +4434,21 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         double fl = 0.0;
         for (int i = 0; i < nbig; ++i)
             if (mats[i].pad == 2) fl += 2.0 * (double)tasks[i].n * tasks[i].n * tasks[i].n;
-        LaunchTimer lt(K_SBR_BT, c->stream, fl, 0.0);
-        hipLaunchKernelGGL(k_sbr_bt<SBT_BL>, dim3(sbt.size()), dim3(64), 0, c->stream, d_mats, (const BtItem *)(db + o_sbt));
+        double by = 0.0;
+        for (const BtItem &b : sbt)
+            if (b.c0 == 0) by += 16.0 * tasks[b.mat].n * (double)tasks[b.mat].n;
+        {
+            LaunchTimer lt(K_EIG_COPY, c->stream, 0.0, by);
+            hipLaunchKernelGGL(k_sbr_tr, dim3(str.size()), dim3(256), 0, c->stream, d_mats, (const BtItem *)(db + o_str), 1);
+        }
+        {
+            LaunchTimer lt(K_SBR_BT, c->stream, fl, 0.0);
+            hipLaunchKernelGGL(k_sbr_bt<SBT_BL>, dim3(sbt.size()), dim3(64), 0, c->stream, d_mats, (const BtItem *)(db + o_sbt));
+        }
+        {
+            LaunchTimer lt(K_EIG_COPY, c->stream, 0.0, by);
+            hipLaunchKernelGGL(k_sbr_tr, dim3(str.size()), dim3(256), 0, c->stream, d_mats, (const BtItem *)(db + o_str), 0);
+        }
     }
     HIPCHK(c, hipGetLastError());
     if (!wymat.empty()) {
@@ -4381,7 +4473,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             if (s) return s;
             {
                 LaunchTimer lt(K_EIG_BT, c->stream, 0.0, 0.0);
-                hipLaunchKernelGGL(k_wy_tfactor, dim3(L.w1 - L.w0), dim3(WY_NB), 0, c->stream, d_wyitems + L.w0);
+                hipLaunchKernelGGL(k_wy_tfactor, dim3(L.w1 - L.w0), dim3(WY_TF_NT), 0, c->stream, d_wyitems + L.w0);
             }
             HIPCHK(c, hipGetLastError());
             s = wy_gemm(L, 1, 0.0);

@@ -715,6 +715,19 @@ nrg_status nrg_sync(nrg_ctx *c) {
     return NRG_OK;
 }
 
+nrg_status nrg_ctx_release_workspace(nrg_ctx *c) {
+    if (!c) return NRG_ERR_ARG;
+    nrg_status s = nrg_sync(c);
+    if (s) return s;
+    if (c->ews) HIPCHK(c, hipFree(c->ews));
+    c->ews = nullptr;
+    c->ews_doubles = 0;
+    if (c->gpart) HIPCHK(c, hipFree(c->gpart));
+    c->gpart = nullptr;
+    c->gpart_tiles = 0;
+    return NRG_OK;
+}
+
 void nrg_timing_enable(int on) { g_timing = on != 0; }
 
 void nrg_timing_filter(const char *kernel) {
@@ -746,7 +759,12 @@ int nrg_timing_get(nrg_kernel_stat *out, int max) {
 nrg_status nrg_dev_alloc(nrg_ctx *c, size_t bytes, void **dptr) {
     if (!c || !dptr) return NRG_ERR_ARG;
     hipError_t e = hipMalloc(dptr, std::max<size_t>(bytes, 8));
-    if (e != hipSuccess) return fail(c, NRG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        // the refusal is reported here; clear it, or the next launch check
+        // (hipGetLastError) would report it again after the caller recovered
+        (void)hipGetLastError();
+        return fail(c, NRG_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
     return NRG_OK;
 }
 
@@ -835,7 +853,12 @@ nrg_status nrg_scatter_h2d(nrg_ctx *c, const nrg_strided *items, int nitems, con
 nrg_status nrg_host_alloc(nrg_ctx *c, size_t bytes, void **hptr) {
     if (!c || !hptr) return NRG_ERR_ARG;
     hipError_t e = hipHostMalloc(hptr, std::max<size_t>(bytes, 8), hipHostMallocDefault);
-    if (e != hipSuccess) return fail(c, NRG_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    if (e != hipSuccess) {
+        // a refusal the caller may recover from (the engine then spills to
+        // pageable memory): cleared, as in nrg_dev_alloc
+        (void)hipGetLastError();
+        return fail(c, NRG_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
     return NRG_OK;
 }
 

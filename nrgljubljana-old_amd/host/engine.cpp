@@ -250,6 +250,8 @@ struct Engine {
     std::vector<int> schedule;
     size_t sched_pos = 0;
     int spills = 0, restores = 0;
+    // host memory holding spilled eigenvectors: now, and the most at any time
+    size_t host_pinned_bytes = 0, host_pageable_bytes = 0, host_peak_bytes = 0;
     int pass1_N = -1;  // iteration being built by pass 1 (its predecessor holds E_prev)
     std::map<std::string, Stage> stages;
     long long eigenpairs = 0;
@@ -824,16 +826,21 @@ struct Engine {
     }
     // The host copy is pinned while pinned memory lasts (the runtime refuses
     // it beyond a limit of its own, measured at about 66 GB on the MI355X
-    // box), then ordinary pageable memory, which the runtime copies through
-    // staging buffers.
+    // box; spills stop at kPinnedSpillMax so that the small pinned buffers
+    // the library grows later still fit), then ordinary pageable memory,
+    // which the runtime copies through staging buffers.
+    static constexpr size_t kPinnedSpillMax = (size_t)48 << 30;
     void spill(Iter &it) {
         it.host = nullptr;
+        if (host_pinned_bytes + it.bytes > kPinnedSpillMax) pinned_exhausted = true;
         it.host_pinned = !pinned_exhausted && nrg_host_alloc(gpu.ctx, it.bytes, &it.host) == NRG_OK;
         if (!it.host_pinned) {
             pinned_exhausted = true;
             it.host = std::malloc(it.bytes);
             if (!it.host) die("spill: out of host memory for " + std::to_string(it.bytes) + " bytes of eigenvectors");
         }
+        (it.host_pinned ? host_pinned_bytes : host_pageable_bytes) += it.bytes;
+        host_peak_bytes = std::max(host_peak_bytes, host_pinned_bytes + host_pageable_bytes);
         gpu.check(nrg_d2h(gpu.ctx, it.host, it.base, it.bytes), "nrg_d2h");
         it.arena.reset();
         gpu.trim();
@@ -858,6 +865,7 @@ struct Engine {
     }
     void free_host(Iter &it) {
         if (!it.host) return;
+        (it.host_pinned ? host_pinned_bytes : host_pageable_bytes) -= it.bytes;
         if (it.host_pinned) (void)nrg_host_free(gpu.ctx, it.host);
         else std::free(it.host);
         it.host = nullptr;
@@ -866,8 +874,18 @@ struct Engine {
     // Spill resident iterations (the one used last first) until `need` more
     // bytes fit in the budget; iteration `keep` and, in pass 1, its
     // predecessor (E_prev) stay.
-    void make_room(size_t need, int keep) {
-        while (resident + need > budget) {
+    // Besides the budget, the device itself must hold `need` plus a reserve
+    // for the data outside the eigenvector arenas (operators, density
+    // matrices, spectral temporaries, workspaces): HBM left free below that
+    // also spills.
+    static constexpr size_t kHbmReserve = (size_t)8 << 30;
+    size_t free_hbm() {
+        size_t fr = 0, tot = 0;
+        gpu.check(nrg_mem_info(gpu.ctx, &fr, &tot), "nrg_mem_info");
+        return fr + gpu.idle_bytes();
+    }
+    void make_room(size_t need, int keep, size_t reserve = kHbmReserve) {
+        while (resident + need > budget || free_hbm() < need + reserve) {
             int victim = -1;
             size_t far = 0;
             for (int N = 0; N < (int)iters.size(); ++N) {
@@ -884,13 +902,17 @@ struct Engine {
         }
     }
     // iteration N is about to be used by a sweep or by pass 2
-    void use(int N) {
+    // `reserve`: HBM the step will allocate besides the eigenvectors.  A
+    // shell of pass 2 builds its operators over all states (blocks of the
+    // full subspace dimensions, the previous shell's still alive during the
+    // recalculation) and the spectral products: about twelve times the
+    // shell's eigenvector bytes.
+    void use(int N, bool pass2 = false) {
         while (sched_pos < schedule.size() && schedule[sched_pos] != N) ++sched_pos;
         Iter &it = iters[N];
-        if (it.spilled) {
-            make_room(it.bytes, N);
-            restore(it);
-        }
+        const size_t reserve = pass2 ? std::max(kHbmReserve, 12 * it.bytes) : kHbmReserve;
+        make_room(it.spilled ? it.bytes : 0, N, reserve);
+        if (it.spilled) restore(it);
         if (sched_pos < schedule.size()) ++sched_pos;
     }
 
@@ -1309,6 +1331,8 @@ struct Engine {
             }
         }
         pass1_N = -1;
+        // no eigensolves after pass 1: its workspace goes back to the device
+        gpu.check(nrg_ctx_release_workspace(gpu.ctx), "nrg_ctx_release_workspace");
         GSenergy = totalenergy;
         scale_last = SCALE(Nmax);
         if (!opt.quiet) std::cout << "Total energy: " << std::setprecision(16) << totalenergy << std::endl;
@@ -1714,7 +1738,7 @@ struct Engine {
         for (int N = P.Ninit; N < Nmax; ++N) {
             const bool last = (N + 1 == Nmax);
             if (!opt.quiet) std::cout << "Spectra N=" << N << std::endl;
-            use(N);
+            use(N, true);
             Iter &it = iters[N];
             if (fen2.is_open()) {
                 std::map<Invar, std::vector<double>> allv;
@@ -1870,6 +1894,7 @@ struct Engine {
         std::ofstream f(opt.stats);
         f << std::setprecision(10) << "{\"wall_s\": " << wall << ", \"eigenpairs\": " << eigenpairs
           << ", \"Nmax\": " << Nmax << ", \"spills\": " << spills << ", \"restores\": " << restores
+          << ", \"host_spill_peak_bytes\": " << host_peak_bytes << ", \"pinned_exhausted\": " << (pinned_exhausted ? "true" : "false")
           << ", \"stages\": {";
         bool first = true;
         for (auto &kv : stages) {

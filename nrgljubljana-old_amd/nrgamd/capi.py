@@ -12,7 +12,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(PKG, "lib", "libnrg_gpu.so")
 
-EXPORTS = ["nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_sync", "nrg_dev_alloc", "nrg_dev_free",
+EXPORTS = ["nrg_ctx_create", "nrg_ctx_destroy", "nrg_last_error", "nrg_sync", "nrg_ctx_release_workspace", "nrg_dev_alloc", "nrg_dev_free",
            "nrg_h2d", "nrg_d2h", "nrg_d2h_strided", "nrg_zero", "nrg_gather_d2h", "nrg_gather_dev", "nrg_scatter_h2d", "nrg_mem_info", "nrg_host_alloc", "nrg_host_free", "nrg_assemble_batch", "nrg_diag_batch", "nrg_diag_values",
            "nrg_gemm_grouped", "nrg_gemm_grouped_noalloc", "nrg_spec_bin", "nrg_timing_enable", "nrg_timing_filter", "nrg_timing_reset",
            "nrg_timing_get"]
@@ -111,6 +111,9 @@ class Ctx:
 
     def sync(self):
         self.check(self.lib.nrg_sync(self.ctx), "sync")
+
+    def release_workspace(self):
+        self.check(self.lib.nrg_ctx_release_workspace(self.ctx), "release_workspace")
 
     def close(self):
         for p in self.bufs:

@@ -28,6 +28,7 @@ nrg_status nrg_ctx_create(nrg_ctx **c, int) {
 void nrg_ctx_destroy(nrg_ctx *c) { delete c; }
 const char *nrg_last_error(const nrg_ctx *c) { return c->err.c_str(); }
 nrg_status nrg_sync(nrg_ctx *) { return NRG_OK; }
+nrg_status nrg_ctx_release_workspace(nrg_ctx *) { return NRG_OK; }
 nrg_status nrg_dev_alloc(nrg_ctx *, size_t bytes, void **p) {
     *p = std::calloc(std::max<size_t>(bytes, 8), 1);
     return *p ? NRG_OK : NRG_ERR_NOMEM;

@@ -99,6 +99,11 @@ def test_diag_batch_large(ctx):
     out = ctx.diag_batch(mats)
     for A, (W, U) in zip(mats, out):
         _check_eig(A, W, U, tol=2e-13)
+    # the workspace returned to the device and allocated again: the same bits
+    ctx.release_workspace()
+    out2 = ctx.diag_batch(mats)
+    for (W, U), (W2, U2) in zip(out, out2):
+        assert np.array_equal(W, W2) and np.array_equal(U, U2)
 
 
 def test_diag_batch_streaming_teams_with_small_blocks(ctx):
