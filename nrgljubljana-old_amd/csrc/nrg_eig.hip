@@ -2075,6 +2075,11 @@ __global__ __launch_bounds__(TRI_NT) void k_eig_tri_blk(const EigMat *mats, cons
 constexpr int CH_NT = 256;
 constexpr int CH_W = 2 * SBR_B;     // columns in a chaser's ring, entries per band column
 constexpr int CH_DONE = 1 << 30;    // progress of a finished sweep
+// LDS row stride of a chaser's ring (doubles): even, so that the accesses
+// along anti-diagonals (next column, one entry up: the dot products and the
+// two-sided update) step by an odd number of doubles and spread over the
+// banks (an odd stride put them all in one bank)
+constexpr int CH_RS = CH_W + 2;
 constexpr long long CH_SPIN_LIMIT = 1LL << 23;  // polls of ~1 us: seconds, far beyond any wait of a running team
 
 struct SbrJob {
@@ -2137,7 +2142,7 @@ __device__ __forceinline__ bool ch_wait(const int *prog, int need, int *status, 
 }
 
 // one sweep of one chaser
-__device__ __forceinline__ bool ch_sweep(const EigMat &mt, int s, double (*ring)[CH_W + 1], double *vs, double *dl,
+__device__ __forceinline__ bool ch_sweep(const EigMat &mt, int s, double (*ring)[CH_RS], double *vs, double *dl,
                                          double *wq, double *db, double *sc, int *status, int *s_flag) {
     const int n = mt.n, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     double *const band = sbr_band(mt.ws, n);
@@ -2214,29 +2219,40 @@ __device__ __forceinline__ bool ch_sweep(const EigMat &mt, int s, double (*ring)
         __syncthreads();
         const double tau = sc[0];
         if (tau != 0.0) {
-            // dot products: left columns (c, r0), D v, below rows [r1, r2)
+            // dot products: left columns (c, r0), D v, below rows [r1, r2),
+            // one wavefront each (three SIMDs at once), 32 terms unrolled
+            // with the terms past len skipped by a select (ring entries
+            // there may hold anything)
             const int nl = r0 - c - 1;
-            if (tid < SBR_B) {
-                if (tid < nl) {
-                    const int j = c + 1 + tid;
-                    double t = 0.0;
-                    for (int u = 0; u < len; ++u) t = fma(vs[u], L(j, r0 + u - j), t);
-                    dl[tid] = t;
+            if (w == 0 && lane < nl) {
+                const int j = c + 1 + lane;
+                double t = 0.0;
+#pragma unroll
+                for (int u = 0; u < SBR_B; ++u) {
+                    const double x = L(j, r0 + min(u, len - 1) - j);
+                    t = u < len ? fma(vs[u], x, t) : t;
                 }
-            } else if (tid < 2 * SBR_B) {
-                const int t = tid - SBR_B;
-                if (t < len) {
-                    double a = 0.0;
-                    for (int u = 0; u < len; ++u) a = fma(t >= u ? L(r0 + u, t - u) : L(r0 + t, u - t), vs[u], a);
-                    wq[t] = tau * a;  // p = tau D v
+                dl[lane] = t;
+            } else if (w == 1 && lane < len) {
+                const int t = lane;
+                double a = 0.0;
+#pragma unroll
+                for (int u = 0; u < SBR_B; ++u) {
+                    const int uu = min(u, len - 1);
+                    const double x = t >= uu ? L(r0 + uu, t - uu) : L(r0 + t, uu - t);
+                    a = u < len ? fma(x, vs[u], a) : a;
                 }
-            } else if (tid < 3 * SBR_B) {
-                const int t = tid - 2 * SBR_B;
-                if (t < nb) {
-                    double a = 0.0;
-                    for (int u = 0; u < len; ++u) a = fma(L(r0 + u, r1 + t - r0 - u), vs[u], a);
-                    db[t] = a;
+                wq[t] = tau * a;  // p = tau D v
+            } else if (w == 2 && lane < nb) {
+                const int t = lane;
+                double a = 0.0;
+#pragma unroll
+                for (int u = 0; u < SBR_B; ++u) {
+                    const int uu = min(u, len - 1);
+                    const double x = L(r0 + uu, r1 + t - r0 - uu);
+                    a = u < len ? fma(x, vs[u], a) : a;
                 }
+                db[t] = a;
             }
             __syncthreads();
             if (w == 0) {  // q = p - (tau / 2) (v . p) v
@@ -2245,16 +2261,19 @@ __device__ __forceinline__ bool ch_sweep(const EigMat &mt, int s, double (*ring)
                 if (lane < len) wq[lane] -= K * vs[lane];
             }
             __syncthreads();
-            const int nL = nl * len, nD = len * len, nB = nb * len;
-            for (int e = tid; e < nL + nD + nB; e += CH_NT) {
-                if (e < nL) {
-                    const int jj = e / len, u = e % len, j = c + 1 + jj;
-                    L(j, r0 + u - j) -= tau * dl[jj] * vs[u];
-                } else if (e < nL + nD) {
-                    const int f = e - nL, t = f / len, u = f % len;
+            // rows of 32 entries: left columns, D, below rows (u = e mod 32)
+            const int nrow = nl + len + nb;
+            for (int e = tid; e < nrow * SBR_B; e += CH_NT) {
+                const int R = e / SBR_B, u = e % SBR_B;
+                if (u >= len) continue;
+                if (R < nl) {
+                    const int j = c + 1 + R;
+                    L(j, r0 + u - j) -= tau * dl[R] * vs[u];
+                } else if (R < nl + len) {
+                    const int t = R - nl;
                     if (t >= u) L(r0 + u, t - u) -= vs[t] * wq[u] + wq[t] * vs[u];
                 } else {
-                    const int f = e - nL - nD, t = f / len, u = f % len;
+                    const int t = R - nl - len;
                     L(r0 + u, r1 + t - r0 - u) -= tau * db[t] * vs[u];
                 }
             }
@@ -2276,7 +2295,7 @@ __device__ __forceinline__ bool ch_sweep(const EigMat &mt, int s, double (*ring)
 
 __global__ __launch_bounds__(CH_NT) void k_sbr_chase(const EigMat *mats, const SbrJob *jobs, const int *tbase, int njobs,
                                                      int *ticket, int *status) {
-    __shared__ double ring[CH_W][CH_W + 1];
+    __shared__ double ring[CH_W][CH_RS];
     __shared__ double vs[SBR_B], dl[SBR_B], wq[SBR_B], db[SBR_B], sc[4];
     __shared__ int s_t, s_flag;
     const int total = tbase[njobs];
