@@ -1364,10 +1364,11 @@ __device__ __noinline__ void tri_blk(const EigMat &mt, int rank, int T, int *bar
 // ---------------------------------------------------------------------------
 // 1e. two-stage reduction, stage 1 (sbr_blk): dense -> symmetric band of
 //     half-width SBR_B (successive band reduction; the band then goes to
-//     tridiagonal form by bulge chasing, section 1f).  The whole symmetric
-//     matrix is kept (the upper triangle is mirrored into the lower one at the
-//     start), so a block of 64 rows holds every element of its rows; team
-//     rank blk_owner(B) alone touches the rows of block B.  Panel p (columns
+//     tridiagonal form by bulge chasing, section 1f).  The lower triangle is
+//     kept up to date (the upper one is mirrored into it at the start), in
+//     64 x 64 tiles with the diagonal tiles whole; team rank blk_owner(B)
+//     alone writes the rows of block B, and tiles above the diagonal are read
+//     as the transposes of their mirror tiles.  Panel p (columns
 //     j0 = SBR_B p .. j0 + SBR_B - 1, trailing range [c0 = j0 + SBR_B, n)):
 //       QR of the panel's lower copy A[c0:n, j0:j0+SBR_B] with its rows
 //         spread over their owners (registers, lane = row): per reflector one
@@ -1377,7 +1378,7 @@ __device__ __noinline__ void tri_blk(const EigMat &mt, int rank, int T, int *bar
 //         partials V^T V and V^T Y -> barrier -> every rank forms T (dlarft)
 //         and S = T^T (V^T Y) T and writes W = Y T - V S / 2 for its rows
 //         -> barrier;
-//       A22 -= V W^T + W V^T on the owned tile rows (both triangles, MFMA).
+//       A22 -= V W^T + W V^T on the owned tile rows (lower triangle, MFMA).
 //     Output: row j0 + k of S2 holds R at columns c0 .. c0 + k (beta at
 //     c0 + k) and reflector k at columns c0 + k + 1 .. n - 1 (implicit 1 at
 //     c0 + k); tau in V_TAU; A's upper triangle the rest of the band
@@ -1436,7 +1437,10 @@ static_assert(sizeof(double) * SB_END <= TRI_COOP_LDS, "stage 1 fits the team la
 // w computes the 16 x 16 block (w & 3, (w >> 2) & 1) over half w >> 3 of each
 // tile's 64 columns (v_mfma_f64_16x16x4f64: A[i][k] in lane i + 16 k, B[k][j]
 // in lane j + 16 k, D[i][j] in lane j + 16 (i % 4), element i / 4); the
-// halves are added in order.  Rows of C are the rank's own (plain loads).
+// halves are added in order.  Only the lower triangle (with whole diagonal
+// tiles) is kept up to date: tiles R <= C are the rank's own rows (plain
+// loads), tiles R > C are read as the transpose of tile (R, C), another
+// rank's rows (agent-scope loads of its write-through stores).
 __device__ __noinline__ void sbr_symm_row(const double *A_, int n_, int npad_, int nt_, int C_, int Bt_,
                                           const double *Vb_, double *Yb_) {
     extern __shared__ double lds[];
@@ -1454,21 +1458,30 @@ __device__ __noinline__ void sbr_symm_row(const double *A_, int n_, int npad_, i
     // which serialises the loads and blurs the wait counts of the ring)
 #define SY_LOAD(S, R)                                                                           \
     if ((R) < nt) {                                                                             \
-        _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                         \
-            const int e = tid + TRI_NT * u, rw = 64 * C + (e >> 6), cl = 64 * (R) + (e & 63);    \
-            const double f = (rw < n && cl < n) ? 1.0 : 0.0;                                    \
-            ra[S][u] = f * gA[(size_t)min(rw, n - 1) * n + min(cl, n - 1)];                     \
+        if ((R) <= C) {                                                                         \
+            _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                     \
+                const int e = tid + TRI_NT * u, rw = 64 * C + (e >> 6), cl = 64 * (R) + (e & 63); \
+                const double f = (rw < n && cl < n) ? 1.0 : 0.0;                                \
+                ra[S][u] = f * gA[(size_t)min(rw, n - 1) * n + min(cl, n - 1)];                 \
+            }                                                                                   \
+        } else {                                                                                \
+            _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                     \
+                const int e = tid + TRI_NT * u, rw = 64 * (R) + (e >> 6), cl = 64 * C + (e & 63); \
+                const double f = (rw < n && cl < n) ? 1.0 : 0.0;                                \
+                ra[S][u] = f * ld_dev((const double *)(gA + (size_t)min(rw, n - 1) * n + min(cl, n - 1))); \
+            }                                                                                   \
         }                                                                                       \
         _Pragma("unroll") for (int u = 0; u < 2; ++u) {                                         \
             const int e = tid + TRI_NT * u;                                                     \
             rv[S][u] = ld_dev(Vb + (size_t)(e >> 6) * npad + 64 * (R) + (e & 63));               \
         }                                                                                       \
     }
-#define SY_PUT(S, BUF)                                                                          \
+#define SY_PUT(S, BUF, R)                                                                       \
     {                                                                                           \
+        const bool tr_ = (R) > C;                                                               \
         _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                         \
             const int e = tid + TRI_NT * u;                                                     \
-            lds[SB_LA + (BUF) * 64 * 65 + (e >> 6) * 65 + (e & 63)] = ra[S][u];                 \
+            lds[SB_LA + (BUF) * 64 * 65 + (tr_ ? (e & 63) * 65 + (e >> 6) : (e >> 6) * 65 + (e & 63))] = ra[S][u]; \
         }                                                                                       \
         _Pragma("unroll") for (int u = 0; u < 2; ++u) {                                         \
             const int e = tid + TRI_NT * u;                                                     \
@@ -1478,7 +1491,7 @@ __device__ __noinline__ void sbr_symm_row(const double *A_, int n_, int npad_, i
 #define SY_STEP(J)                                                                              \
     if (i0_ + (J) < cnt) {                                                                      \
         const int R = Bt + i0_ + (J), buf = (i0_ + (J)) & 1;                                    \
-        if (R + 1 < nt) SY_PUT(((J) + 1) & 3, buf ^ 1)                                          \
+        if (R + 1 < nt) SY_PUT(((J) + 1) & 3, buf ^ 1, R + 1)                                   \
         SY_LOAD(((J) + 3) & 3, R + 3)                                                           \
         _Pragma("unroll") for (int s = 8 * h; s < 8 * h + 8; ++s) {                             \
             const double a = lds[SB_LA + buf * 64 * 65 + (16 * bi + (lane & 15)) * 65 + 4 * s + (lane >> 4)]; \
@@ -1490,7 +1503,7 @@ __device__ __noinline__ void sbr_symm_row(const double *A_, int n_, int npad_, i
     SY_LOAD(0, Bt)
     SY_LOAD(1, Bt + 1)
     SY_LOAD(2, Bt + 2)
-    SY_PUT(0, 0)
+    SY_PUT(0, 0, Bt)
     __syncthreads();
     SB_RING4(SY_STEP)
 #undef SY_LOAD
@@ -1513,7 +1526,9 @@ __device__ __noinline__ void sbr_symm_row(const double *A_, int n_, int npad_, i
 }
 
 // A[r][c] -= sum_k V[r][k] W[c][k] + W[r][k] V[c][k] on the trailing tile
-// row of block C, rows and columns >= c0 (both triangles).  The row
+// row of block C, rows and columns >= c0, tiles R <= C (the lower triangle
+// and the whole diagonal tile; stores write through, since other ranks read
+// these tiles transposed).  The row
 // operands -[V W] of block C are staged once (k-major); per tile R the
 // column operands [W V] go through a double LDS buffer and the old tile is
 // the MFMA accumulator of wavefront w's 16 x 16 block (w / 4, w % 4).
@@ -1526,7 +1541,7 @@ __device__ __noinline__ void sbr_syr2k_row(double *A_, int n_, int npad_, int nt
     const int tid = threadIdx.x, w = uni(tid >> 6), lane = tid & 63;
     const int bi = w >> 2, bj = w & 3;
     const int row0 = 64 * C + 16 * bi + (lane >> 4);  // element v: row row0 + 4 v
-    const int cnt = nt - Bt;
+    const int cnt = C - Bt + 1;  // tiles R = Bt .. C (the lower triangle)
     double *const Ls = lds + SB_LS;
     __syncthreads();
 #pragma unroll
@@ -1571,7 +1586,7 @@ __device__ __noinline__ void sbr_syr2k_row(double *A_, int n_, int npad_, int nt
         const int col = 64 * R + 16 * bj + (lane & 15);                                         \
         _Pragma("unroll") for (int v = 0; v < 4; ++v) {                                         \
             const int r = row0 + 4 * v;                                                         \
-            if (r >= c0 && r < n && col >= c0 && col < n) gA[(size_t)r * n + col] = acc[v];    \
+            if (r >= c0 && r < n && col >= c0 && col < n) st_dev((double *)(gA + (size_t)r * n + col), acc[v]); \
         }                                                                                       \
         __syncthreads();                                                                        \
     }
@@ -1659,7 +1674,7 @@ __device__ __noinline__ void sbr_mirror(const SbrCtx *sc) {
             for (int u = 0; u < 4; ++u) {
                 const int e = tid + TRI_NT * u, ii = e >> 6, cc = e & 63;  // A[64 B + ii][64 Cp + cc]
                 const int r = 64 * B + ii, c = 64 * Cp + cc;
-                if (r < n && c < n && c < r) gA[(size_t)r * n + c] = La[cc * 65 + ii];
+                if (r < n && c < n && c < r) st_dev((double *)(gA + (size_t)r * n + c), La[cc * 65 + ii]);
             }
             __syncthreads();
         }
@@ -2094,7 +2109,8 @@ __host__ __device__ inline int *sbr_prog(double *ws, int n) { return (int *)(sbr
 
 // After stage 1: the band (row j, columns j .. j + SBR_B) into the band
 // buffer, taking a panel row's columns from the panel's trailing start on
-// from S2; the panel rows' R and reflectors copied from S2 into A (S2 is
+// from S2 and the rest from A's lower triangle (A[j + d][j]); the panel
+// rows' R and reflectors copied from S2 into A's upper triangle (S2 is
 // the divide-and-conquer workspace later); progress counters zeroed.
 // Item: (matrix, 16 rows).
 __global__ __launch_bounds__(256) void k_sbr_band(const EigMat *mats, const BtItem *items) {
@@ -2111,7 +2127,7 @@ __global__ __launch_bounds__(256) void k_sbr_band(const EigMat *mats, const BtIt
         const int c0j = (j / SBR_B + 1) * SBR_B;  // trailing start of row j's panel
         const int col = j + d;
         double v = 0.0;
-        if (d <= SBR_B && col < n) v = (j < nr && col >= c0j) ? P2[(size_t)j * n + col] : mt.A[(size_t)j * n + col];
+        if (d <= SBR_B && col < n) v = (j < nr && col >= c0j) ? P2[(size_t)j * n + col] : mt.A[(size_t)col * n + j];
         band[(size_t)j * CH_W + d] = v;
         if (d == 0) prog[j] = 0;
     }
@@ -2320,6 +2336,21 @@ __global__ __launch_bounds__(CH_NT) void k_sbr_chase(const EigMat *mats, const S
     }
 }
 
+// Raw buffer access (vector memory instructions: a scalar resource, the
+// lane's 32-bit offset and a scalar offset per access, so a run of loads
+// from uniform rows needs one address register).  Offsets at or past
+// `bytes` read 0 and drop stores (range-checked by the hardware).
+typedef unsigned int bt_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, int vo, int so) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, int vo, int so, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(bt_u32x2, v), r, vo, so, 0);
+}
+
 // Two-stage matrices: the eigenvectors of the tridiagonal between Vc (row v
 // = vector v) and an entry-major copy Zt in S2 (row p = entry p of every
 // vector), in 64 x 64 tiles through LDS (item: matrix, tile ti * nt + tj of
@@ -2373,7 +2404,11 @@ __global__ __launch_bounds__(256) void k_sbr_tr(const EigMat *mats, const BtItem
 // sharing one staging 1.68 s, reflectors a step ahead in registers 1.70 s
 // (occupancy): the reflectors' LDS broadcast reads bound a step, not HBM.
 constexpr int SBT_BL = 48;  // sweeps per block of the stage-2 back-transformation
-template <int BL>
+// PF: the next step's new window entries are loaded while this step
+// computes (measured on the keep-10000 block set: 0.82 s with, 0.85 s
+// without; capping the registers at two wavefronts per SIMD spills and
+// takes 2.4-3.3 s)
+template <int BL, bool PF>
 __global__ __launch_bounds__(64) void k_sbr_bt(const EigMat *mats, const BtItem *items) {
     constexpr int WIN = SBR_B + BL - 1;  // window length
     __shared__ double rv[2][BL][SBR_B];
@@ -2385,13 +2420,19 @@ __global__ __launch_bounds__(64) void k_sbr_bt(const EigMat *mats, const BtItem 
     const bool act = cv < n;
     // the vectors entry-major (Zt row p = entry p of every vector), so that a
     // window load or store of the 64 lanes is one contiguous 512-byte run
-    gdouble *const Zt = (gdouble *)(mt.ws + 2 * (size_t)n * n);
-    const int cz = act ? cv : 0;
-    auto zt = [&](int p) -> gdouble & { return Zt[(size_t)min(p, n - 1) * n + cz]; };
+    double *const Zt = mt.ws + 2 * (size_t)n * n;
+    // window rows through a buffer resource based at the step's first row:
+    // rows past n read 0, and the lanes past n (offset 2^31, beyond the
+    // resource's 1 GiB) neither read nor write
+    const int vo = act ? cv * 8 : (int)0x80000000u;
+    auto rows_from = [&](int base) {
+        return buf_rsrc(Zt + (size_t)base * n, (unsigned)min<size_t>((size_t)(n - base) * n * 8, (size_t)1 << 30));
+    };
+    const int rb = n * 8;  // bytes per row
     const gdouble *const A = (const gdouble *)mt.A;
     const int nsw = n - 1;  // sweeps 0 .. n - 2
     const int nblk = (nsw + BL - 1) / BL;
-    double z[WIN], zn[SBR_B];
+    double z[WIN], zn[PF ? SBR_B : 1];
     // reflectors H(s, k), s in [s0, s1) (row n-1-s, columns k b ..; tau in
     // place of the unit first element) into LDS buffer buf
     // (loads issued in batches of RB before their LDS stores: a load and
@@ -2427,20 +2468,19 @@ __global__ __launch_bounds__(64) void k_sbr_bt(const EigMat *mats, const BtItem 
     for (int bi = nblk - 1; bi >= 0; --bi) {
         const int s0 = bi * BL, s1 = min(s0 + BL, nsw);
         int base = s0 + 1;
+        {
+            const __amdgpu_buffer_rsrc_t rs = rows_from(base);
 #pragma unroll
-        for (int p = 0; p < WIN; ++p) {
-            const double v = zt(base + p);
-            z[p] = (act && base + p < n) ? v : 0.0;
+            for (int p = 0; p < WIN; ++p) z[p] = buf_ld(rs, vo, p * rb);
         }
         stage_refl(s0, s1, 0, 0);
         wave_lds_sync();
         for (int k = 0, buf = 0; base < n; ++k, base += SBR_B, buf ^= 1) {
             const bool more = base + SBR_B < n;
+            const __amdgpu_buffer_rsrc_t rs = rows_from(base);
+            if (PF) {
 #pragma unroll
-            for (int p = 0; p < SBR_B; ++p) {
-                const int idx = base + WIN + p;
-                const double v = zt(idx);
-                zn[p] = (act && more && idx < n) ? v : 0.0;
+                for (int p = 0; p < SBR_B; ++p) zn[PF ? p : 0] = buf_ld(rs, vo, (WIN + p) * rb);
             }
 #pragma unroll
             for (int o = BL - 1; o >= 0; --o) {
@@ -2463,12 +2503,12 @@ __global__ __launch_bounds__(64) void k_sbr_bt(const EigMat *mats, const BtItem 
             wave_lds_sync();
             // entries base .. base + b - 1 are final for this block
 #pragma unroll
-            for (int p = 0; p < SBR_B; ++p)
-                if (act && base + p < n) zt(base + p) = z[p];
+            for (int p = 0; p < SBR_B; ++p) buf_st(rs, vo, p * rb, z[p]);
 #pragma unroll
             for (int p = 0; p < WIN - SBR_B; ++p) z[p] = z[p + SBR_B];
 #pragma unroll
-            for (int p = 0; p < SBR_B; ++p) z[WIN - SBR_B + p] = zn[p];
+            for (int p = 0; p < SBR_B; ++p)
+                z[WIN - SBR_B + p] = PF ? zn[PF ? p : 0] : more ? buf_ld(rs, vo, (WIN + p) * rb) : 0.0;
         }
     }
 }
@@ -4462,7 +4502,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         }
         {
             LaunchTimer lt(K_SBR_BT, c->stream, fl, 0.0);
-            hipLaunchKernelGGL(k_sbr_bt<SBT_BL>, dim3(sbt.size()), dim3(64), 0, c->stream, d_mats, (const BtItem *)(db + o_sbt));
+            hipLaunchKernelGGL((k_sbr_bt<SBT_BL, true>), dim3(sbt.size()), dim3(64), 0, c->stream, d_mats, (const BtItem *)(db + o_sbt));
         }
         {
             LaunchTimer lt(K_EIG_COPY, c->stream, 0.0, by);
