@@ -37,9 +37,9 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and MFMA)
 PEAK_HBM_GBS = 8000.0
 # What bounds each kernel, and which of its counted quantities is algorithmic.
 # k_eig_tri_blk: the band reduction of a block of dimension d reads the
-# trailing matrix once for Y = A V and reads and writes it once for the
-# rank-64 update per panel of 32 columns (d^3 / 4 bytes; the one-stage
-# reduction below the two-stage size, sum_j 4 (d-j)^2); the streaming one
+# trailing matrix once for Y = A V and reads and writes its lower triangle
+# for the rank-64 update per panel of 32 columns (d^3 / 6 bytes; the
+# one-stage reduction below the two-stage size, sum_j 4 (d-j)^2); the streaming one
 # (k_eig_tri_coop) the whole trailing matrix per column; binning / assembly
 # / transfers are byte streams; the contractions run on FP64 MFMA, the
 # bulge-chasing back-transformation (k_sbr_bt, 2 d^3) on the FP64 FMA units

@@ -4107,9 +4107,11 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             double &by = tasks[i].n <= REG_ONE_N ? by1 : regjob[i] ? byr : byt;
             fl += 4.0 / 3.0 * n * n * n;
             // the blocked product reads half the trailing matrix per column;
-            // the two-stage reduction reads and writes the trailing matrix a
-            // few times per panel of 32 columns (3 passes: 8 n^3 / 32 bytes)
-            by += (blkjob[i] == 2 ? 0.25 : blkjob[i] ? 4.0 / 3.0 : 8.0 / 3.0) * n * n * n;
+            // the two-stage reduction, per panel of 32 columns of a trailing
+            // matrix of order m, reads it for the product (8 m^2 bytes) and
+            // reads and writes its lower triangle in the update (8 m^2):
+            // sum over panels 16 m^2 = n^3 / 6 bytes
+            by += (blkjob[i] == 2 ? 1.0 / 6.0 : blkjob[i] ? 4.0 / 3.0 : 8.0 / 3.0) * n * n * n;
         }
         bool any_blk = false;
         for (int i = 0; i < ntasks; ++i) any_blk = any_blk || blkjob[i];
