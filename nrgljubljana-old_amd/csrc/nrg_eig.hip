@@ -181,7 +181,7 @@ constexpr int BAR_STRIDE = 32;                    // ints between team barrier c
 struct TriJob {
     int mat, T;
     int blocked;  // 2: two-stage (sbr_blk), 1: blocked reduction (tri_blk), 0: streaming rows (tri_team)
-    int pad;
+    int next;     // job the same team runs next (a chain of jobs on one team), -1: none
 };
 
 // (global address space, so that they are global_* rather than flat_*
@@ -2047,14 +2047,20 @@ __device__ __forceinline__ void tri_teams(const EigMat *mats, const TriJob *jobs
             if (tbase[mid] <= t0 + t) lo = mid;
             else hi = mid;
         }
-        const TriJob jb = jobs[lo];
-        const EigMat mt = mats[jb.mat];
         const int rank = t0 + t - tbase[lo];
-        if (jb.blocked == 2) sbr_blk(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag);
-        else if (jb.blocked) tri_blk(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, &s_flag);
-        else if (mt.n <= TEAM_LDS_N) tri_team<true>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, red, &s_flag);
-        else tri_team<false>(mt, rank, jb.T, bars + BAR_STRIDE * lo, status, red, &s_flag);
-        __syncthreads();
+        // the job, then the jobs chained after it on the same team (same
+        // size, same ranks: no wait for members to arrive)
+        for (int cur = lo; cur >= 0;) {
+            const TriJob jb = jobs[cur];
+            const EigMat mt = mats[jb.mat];
+            int *const bar = bars + BAR_STRIDE * cur;
+            if (jb.blocked == 2) sbr_blk(mt, rank, jb.T, bar, status, &s_flag);
+            else if (jb.blocked) tri_blk(mt, rank, jb.T, bar, status, &s_flag);
+            else if (mt.n <= TEAM_LDS_N) tri_team<true>(mt, rank, jb.T, bar, status, red, &s_flag);
+            else tri_team<false>(mt, rank, jb.T, bar, status, red, &s_flag);
+            __syncthreads();
+            cur = jb.next;
+        }
     }
     }
 }
@@ -3878,6 +3884,10 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     std::vector<int> teamT(ntasks, 1);
     std::vector<double> cost(ntasks);
     std::vector<char> global(ntasks, 0);
+    // chains of two-stage jobs on one team: the next job of a task, and
+    // whether a task is a chain's continuation (not a queue job)
+    std::vector<int> chain_next(ntasks, -1);
+    std::vector<char> chained(ntasks, 0);
     std::vector<char> regjob(ntasks, 0), blkjob(ntasks, 0);
     const int blk_min_n = c->blk_min_n, blk_tdiv = c->blk_tdiv;
     for (int i = 0; i < ntasks; ++i) {
@@ -4002,6 +4012,72 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             teamT[i] = T;
             cost[i] = blk_o(n) + blk_w(n) / T;
         }
+        // When the two-stage jobs do not all fit on the device at once, a
+        // team drawn from the ticket queue would assemble from workgroups
+        // that finish earlier jobs at different times, the first ones idle
+        // until the last arrives (clock probes: a third of the launch's
+        // CU time).  Instead the jobs form chains on fixed teams: the
+        // largest that fit at once start the chains (their team sizes), and
+        // every further job, largest first, is appended to the chain that
+        // would finish it first, run by that chain's team.
+        std::vector<int> sj;
+        long sumT = 0;
+        for (int i : bj)
+            if (blkjob[i] == 2) {
+                sj.push_back(i);
+                sumT += teamT[i];
+            }
+        if (sumT > tri_grid && sj.size() > 1) {
+            std::stable_sort(sj.begin(), sj.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+            struct Lane {
+                int T, last;
+                double fin;
+            };
+            std::vector<Lane> lanes;
+            int used = 0;
+            size_t k = 0;
+            for (; k < sj.size() && used + teamT[sj[k]] <= tri_grid; ++k) {
+                lanes.push_back({teamT[sj[k]], sj[k], cost[sj[k]]});
+                global[sj[k]] = 1;
+                used += teamT[sj[k]];
+            }
+            // the workgroups left over start one more chain, with the largest
+            // job whose panel rows they can hold
+            if (tri_grid - used >= 2) {
+                for (size_t m = k; m < sj.size(); ++m) {
+                    const int i = sj[m], nt = (tasks[i].n + 63) / 64;
+                    const int T = std::min(teamT[i], tri_grid - used);
+                    if (sbr_owned_max(nt, T) > SBR_MAXQ) continue;
+                    teamT[i] = T;
+                    cost[i] = blk_o(tasks[i].n) + blk_w(tasks[i].n) / T;
+                    lanes.push_back({T, i, cost[i]});
+                    global[i] = 1;
+                    used += T;
+                    sj.erase(sj.begin() + m);
+                    break;
+                }
+            }
+            for (; k < sj.size(); ++k) {
+                const int i = sj[k], nt = (tasks[i].n + 63) / 64;
+                const double n = tasks[i].n;
+                int best = -1;
+                double bf = 0.0;
+                for (int l = 0; l < (int)lanes.size(); ++l) {
+                    if (sbr_owned_max(nt, lanes[l].T) > SBR_MAXQ) continue;
+                    const double f = lanes[l].fin + blk_o(n) + blk_w(n) / lanes[l].T;
+                    if (best < 0 || f < bf) {
+                        best = l;
+                        bf = f;
+                    }
+                }
+                if (best < 0) continue;  // no chain's team suits it: it stays a queue job
+                chain_next[lanes[best].last] = i;
+                chained[i] = 1;
+                teamT[i] = lanes[best].T;
+                lanes[best].last = i;
+                lanes[best].fin = bf;
+            }
+        }
     }
     std::vector<int> order(ntasks);
     for (int i = 0; i < ntasks; ++i) order[i] = i;
@@ -4010,13 +4086,13 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     std::vector<std::vector<int>> queue(N_XCD), rqueue(N_XCD);
     std::vector<double> load(N_XCD, 0.0), rload(N_XCD, 0.0);
     for (int i : order)
-        if (global[i]) {
+        if (global[i] && !chained[i]) {
             gq.push_back(i);
             for (int x = 0; x < N_XCD; ++x) load[x] += cost[i] * teamT[i] / tri_grid;
         }
     std::vector<int> ones;  // one-workgroup matrices
     for (int i : order) {
-        if (global[i]) continue;
+        if (global[i] || chained[i]) continue;
         if (tasks[i].n <= REG_ONE_N) {
             ones.push_back(i);
             continue;
@@ -4032,13 +4108,13 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         tbase.assign(1, 0);
         qjob.assign(1, 0);
         for (int i : g) {
-            jobs.push_back({i, teamT[i], blkjob[i], 0});
+            jobs.push_back({i, teamT[i], blkjob[i], -1});
             tbase.push_back(tbase.back() + teamT[i]);
         }
         qjob.push_back((int)jobs.size());
         for (int x = 0; x < N_XCD; ++x) {
             for (int i : q[x]) {
-                jobs.push_back({i, teamT[i], blkjob[i], 0});
+                jobs.push_back({i, teamT[i], blkjob[i], -1});
                 tbase.push_back(tbase.back() + teamT[i]);
             }
             qjob.push_back((int)jobs.size());
@@ -4048,6 +4124,18 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
     std::vector<int> tbase, qjob, rtbase, rqjob;
     job_list(gq, queue, jobs, tbase, qjob);
     job_list({}, rqueue, rjobs, rtbase, rqjob);
+    // chain continuations after the queues (reached only through `next`)
+    {
+        std::vector<int> jof(ntasks, -1);
+        for (int j = 0; j < (int)jobs.size(); ++j) jof[jobs[j].mat] = j;
+        for (int i = 0; i < ntasks; ++i)
+            if (chained[i]) {
+                jof[i] = (int)jobs.size();
+                jobs.push_back({i, teamT[i], blkjob[i], -1});
+            }
+        for (int j = 0; j < (int)jobs.size(); ++j)
+            if (chain_next[jobs[j].mat] >= 0) jobs[j].next = jof[chain_next[jobs[j].mat]];
+    }
     // counters: tickets + one barrier per job, register teams first; barrier
     // counters a cache line apart (teams polling neighbouring counters would
     // contend for one line)
