@@ -4331,7 +4331,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         lv.s1 = (int)schunks.size();
         lv.u1 = (int)uchunks.size();
         lv.r0 = (int)gred.size();
-        lv.mfma = gemm_plan(gouts, lo0, gks, gitems, gred, gslots);
+        lv.mfma = gemm_plan(gouts, lo0, gks, gitems, gred, gslots, c->gemm_mfma_min_tiles);
         lv.r1 = (int)gred.size();
         lv.g1 = (int)gitems.size();
         lv.x1 = (int)gathers.size();
@@ -4431,7 +4431,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                 if (stage >= 2) L.flops += 2.0 * n * (double)nb * np;
             }
             L.r[stage][0] = (int)wyred.size();
-            L.mfma[stage] = gemm_plan(wyouts, so0, wyks, wygi, wyred, gslots);
+            L.mfma[stage] = gemm_plan(wyouts, so0, wyks, wygi, wyred, gslots, c->gemm_mfma_min_tiles);
             L.r[stage][1] = (int)wyred.size();
             L.g[stage][1] = (int)wygi.size();
         }

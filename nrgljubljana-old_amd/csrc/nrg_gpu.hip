@@ -491,11 +491,11 @@ __global__ __launch_bounds__(256) void k_gemm_mfma(const nrg_gemm_out *outs, con
 }
 
 bool gemm_plan(const std::vector<nrg_gemm_out> &outs, size_t o0, const std::vector<int> &ksteps,
-               std::vector<GemmItem> &items, std::vector<GemmItem> &reduce, int &slots) {
+               std::vector<GemmItem> &items, std::vector<GemmItem> &reduce, int &slots, int min_tiles) {
     size_t big = 0;
     for (size_t o = o0; o < outs.size(); ++o)
         big += (size_t)((outs[o].m + GTM - 1) / GTM) * ((outs[o].n + GTM - 1) / GTM);
-    if (big >= (size_t)GEMM_MFMA_MIN_TILES) {
+    if (big >= (size_t)std::max(1, min_tiles)) {
         for (size_t o = o0; o < outs.size(); ++o)
             for (int tm = 0; tm * GTM < outs[o].m; ++tm)
                 for (int tn = 0; tn * GTM < outs[o].n; ++tn) items.push_back({(int)o, tm, tn, 0, 0, 0});
@@ -647,6 +647,7 @@ nrg_status nrg_ctx_create(nrg_ctx **out, int device) {
     if (!out) return NRG_ERR_ARG;
     nrg_ctx *c = new nrg_ctx();
     c->device = device;
+    if (const char *t = getenv("NRG_GEMM_MFMA_MIN_TILES")) c->gemm_mfma_min_tiles = std::max(1, atoi(t));
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
@@ -944,7 +945,7 @@ static nrg_status gemm_prepare(nrg_ctx *c, const nrg_gemm_out *outs, int nouts, 
     }
     std::vector<GemmItem> reduce;
     L.mfma = gemm_plan(std::vector<nrg_gemm_out>(outs, outs + nouts), 0, gemm_ksteps(outs, nouts, terms), items,
-                       reduce, L.slots);
+                       reduce, L.slots, c->gemm_mfma_min_tiles);
     L.ob = sizeof(nrg_gemm_out) * nouts;
     L.tb = sizeof(nrg_gemm_term) * nterms;
     L.ib = sizeof(GemmItem) * items.size();

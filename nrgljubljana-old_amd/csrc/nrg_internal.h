@@ -51,6 +51,9 @@ struct nrg_ctx {
     int sbr = 1;
     // co-resident bulge-chasing workgroups
     int chase_capacity = 0;
+    // grouped GEMM: launches with at least this many 64 x 64 output tiles run
+    // on FP64 MFMA (NRG_GEMM_MFMA_MIN_TILES at creation)
+    int gemm_mfma_min_tiles = 256;
     // grouped GEMM: partial tiles of split outputs
     double *gpart = nullptr;
     size_t gpart_tiles = 0;
@@ -257,13 +260,12 @@ nrg_status gemm_reserve(nrg_ctx *c, int slots);
 nrg_status gemm_launch_dev(nrg_ctx *c, int kid, const nrg_gemm_out *d_outs, const nrg_gemm_term *d_terms,
                            const GemmItem *d_items, int nitems, const GemmItem *d_reduce, int nreduce,
                            double flops, double bytes, bool mfma = false, double *part = nullptr);
-// Launches with at least GEMM_MFMA_MIN_TILES tiles of GTM x GTM run on FP64
-// MFMA (k_gemm_mfma, tiles never split); smaller ones on the 32 x 32 FMA tile
-// (k_gemm, split when few).  gemm_plan appends the work items of the outputs
-// [o0, outs.size()) in the chosen tiling (and the reduce items of split
-// tiles, raising `slots` to the partial slots they need) and returns true
-// for the MFMA kernel.
+// Launches with at least `min_tiles` tiles of GTM x GTM (the context's
+// gemm_mfma_min_tiles) run on FP64 MFMA (k_gemm_mfma, tiles never split);
+// smaller ones on the 32 x 32 FMA tile (k_gemm, split when few).  gemm_plan
+// appends the work items of the outputs [o0, outs.size()) in the chosen
+// tiling (and the reduce items of split tiles, raising `slots` to the
+// partial slots they need) and returns true for the MFMA kernel.
 constexpr int GTM = 64;
-constexpr int GEMM_MFMA_MIN_TILES = 256;
 bool gemm_plan(const std::vector<nrg_gemm_out> &outs, size_t o0, const std::vector<int> &ksteps,
-               std::vector<GemmItem> &items, std::vector<GemmItem> &reduce, int &slots);
+               std::vector<GemmItem> &items, std::vector<GemmItem> &reduce, int &slots, int min_tiles);

@@ -53,6 +53,7 @@ static int nrg_main_args(int argc, char **argv) {
             return 0;
         } else nrg::die("unknown option " + a);
     }
+    nrg::load_tuning(opt.tables_dir);
     return nrg::run(opt);
 }
 

@@ -13,6 +13,21 @@ namespace nrg {
 
 void die(const std::string &msg) { throw Error(msg); }
 
+void load_tuning(const std::string &tables_dir) {
+    static const std::set<std::string> known = {"NRG_BLK_MIN_N", "NRG_GEMM_MFMA_MIN_TILES"};
+    std::ifstream f(tables_dir + "/tune.txt");
+    std::string line;
+    while (f && std::getline(f, line)) {
+        const size_t h = line.find('#');
+        if (h != std::string::npos) line.resize(h);
+        std::istringstream is(line);
+        std::string name, value;
+        if (!(is >> name >> value)) continue;
+        if (!known.count(name)) die("tune.txt: unknown setting " + name);
+        setenv(name.c_str(), value.c_str(), 0);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // param: a [param] block of keyword=value lines; duplicates are fatal,
 // unknown keywords are reported (src/misc.h:71-124, src/nrg.cc:681-703).

@@ -33,6 +33,12 @@ struct Stop : Error {
     explicit Stop(const std::string &m) : Error(m) {}
 };
 [[noreturn]] void die(const std::string &msg);
+// Size thresholds measured on the device by nrgamd/autotune.py, read from
+// `<tables_dir>/tune.txt` ("NAME value" lines, # comments) into environment
+// variables of the same names that are not set already (the library reads
+// them when a context is created or first eigensolves).  A missing file
+// leaves the built-in defaults.
+void load_tuning(const std::string &tables_dir);
 
 // ---- parameters (src/param.cc defaults, src/misc.h:71-124 parser) --------
 struct Params {

@@ -329,6 +329,7 @@ static int nrg_serve_main_args(int argc, char **argv) {
     }
     if (req.empty()) nrg::die("nrg-serve: no request list");
     if (tables.empty()) nrg::die("nrg-serve: --tables or NRG_TABLES required");
+    nrg::load_tuning(tables);
     return nrg::serve(req, report, tables, device, slots, window, memf);
 }
 

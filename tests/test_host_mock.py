@@ -150,3 +150,26 @@ def test_error_returns_status_without_exit(tmp_path):
         f.write("# not a data file\n")
     r = _run_rc(wd)
     assert r.returncode == 1 and "Error:" in r.stderr and "version" in r.stderr
+
+
+def test_tuning_table_read_and_checked(tmp_path):
+    # tables/tune.txt (written by nrgamd.autotune on the device) is read at
+    # start: known settings go to the environment, anything else is an error
+    import shutil
+    import subprocess
+    wd = str(tmp_path / "run")
+    os.makedirs(wd)
+    nrgio.make_case(wd, sym="QS", nch=1, Nmax=4, keep=40, keepenergy=6.0, T=0.05)
+    nrgio.ensure_tables()
+    tables = str(tmp_path / "tables")
+    shutil.copytree(nrgio.TABLES, tables)
+    binary = nrgio.build_mock()
+    env = dict(os.environ, NRG_TABLES=tables)
+    with open(os.path.join(tables, "tune.txt"), "w") as f:
+        f.write("# measured\nNRG_BLK_MIN_N 1280  # two-stage from here\nNRG_GEMM_MFMA_MIN_TILES 256\n")
+    r = subprocess.run([binary], cwd=wd, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    with open(os.path.join(tables, "tune.txt"), "w") as f:
+        f.write("NRG_NO_SUCH_KNOB 3\n")
+    r = subprocess.run([binary], cwd=wd, env=env, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "unknown setting NRG_NO_SUCH_KNOB" in r.stderr
