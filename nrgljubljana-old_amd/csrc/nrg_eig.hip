@@ -1427,10 +1427,22 @@ static_assert(sizeof(double) * SB_END <= TRI_COOP_LDS, "stage 1 fits the team la
 // tile row of an owned block C: tile R's operands are loaded three tiles
 // ahead into a four-deep register ring, staged in one of two LDS buffers, and
 // one barrier per tile separates the staging from the MFMAs; so about three
-// tiles of loads are in flight while a tile is computed.
-#define SB_RING4(STEP)                          \
-    for (int i0_ = 0; i0_ < cnt; i0_ += 4) {    \
-        STEP(0) STEP(1) STEP(2) STEP(3)         \
+// tiles of loads are in flight while a tile is computed.  The loads and the
+// staging of every step are unconditional (a load past the row reads one
+// fixed address, a staging past the row fills the idle buffer), and the
+// steps run in groups of four without guards, the last group with them: a
+// skipped load or staging in the loop body would make the compiler assume
+// the ring's loads may still be pending at every later step and wait for
+// them all before each new load.
+#define SB_RING4(STEP)                                                          \
+    {                                                                           \
+        int i0_ = 0;                                                            \
+        for (; i0_ + 4 <= cnt; i0_ += 4) {                                      \
+            STEP(0, true) STEP(1, true) STEP(2, true) STEP(3, true)             \
+        }                                                                       \
+        if (i0_ < cnt) {                                                        \
+            STEP(0, true) STEP(1, i0_ + 1 < cnt) STEP(2, i0_ + 2 < cnt)         \
+        }                                                                       \
     }
 
 // Y[C] = sum_R A[C rows][R columns] V[R] over R in [Bt, nt) (64 x 32): wavefront
@@ -1438,9 +1450,10 @@ static_assert(sizeof(double) * SB_END <= TRI_COOP_LDS, "stage 1 fits the team la
 // tile's 64 columns (v_mfma_f64_16x16x4f64: A[i][k] in lane i + 16 k, B[k][j]
 // in lane j + 16 k, D[i][j] in lane j + 16 (i % 4), element i / 4); the
 // halves are added in order.  Only the lower triangle (with whole diagonal
-// tiles) is kept up to date: tiles R <= C are the rank's own rows (plain
-// loads), tiles R > C are read as the transpose of tile (R, C), another
-// rank's rows (agent-scope loads of its write-through stores).
+// tiles) is kept up to date: tiles R <= C are the rank's own rows, tiles
+// R > C are read as the transpose of tile (R, C), another rank's rows (all
+// through agent-scope loads, which see the other ranks' write-through
+// stores; no tile is read twice by one rank in a panel).
 __device__ __noinline__ void sbr_symm_row(const double *A_, int n_, int npad_, int nt_, int C_, int Bt_,
                                           const double *Vb_, double *Yb_) {
     extern __shared__ double lds[];
@@ -1453,45 +1466,44 @@ __device__ __noinline__ void sbr_symm_row(const double *A_, int n_, int npad_, i
     const int cnt = nt - Bt;
     double ra[4][4], rv[4][2];
     blk_d4 acc = {0.0, 0.0, 0.0, 0.0};
-    // loads outside the matrix read a clamped address and are zeroed by a
-    // factor (a guarded select would become a branch of its own with a wait,
-    // which serialises the loads and blurs the wait counts of the ring)
+    // Loads outside the matrix read a clamped address; the staging multiplies
+    // them by zero.  The ring registers hold the raw loaded values, so the
+    // loads stay in flight until their tile is staged (a factor applied at the
+    // load, or a guarded load, makes the compiler wait for each load at once).
+    // Tiles R > C are the transposes of tile (R, C): the same agent-scope load
+    // from the mirrored address, transposed when staged.
 #define SY_LOAD(S, R)                                                                           \
-    if ((R) < nt) {                                                                             \
-        if ((R) <= C) {                                                                         \
-            _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                     \
-                const int e = tid + TRI_NT * u, rw = 64 * C + (e >> 6), cl = 64 * (R) + (e & 63); \
-                const double f = (rw < n && cl < n) ? 1.0 : 0.0;                                \
-                ra[S][u] = f * gA[(size_t)min(rw, n - 1) * n + min(cl, n - 1)];                 \
-            }                                                                                   \
-        } else {                                                                                \
-            _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                     \
-                const int e = tid + TRI_NT * u, rw = 64 * (R) + (e >> 6), cl = 64 * C + (e & 63); \
-                const double f = (rw < n && cl < n) ? 1.0 : 0.0;                                \
-                ra[S][u] = f * ld_dev((const double *)(gA + (size_t)min(rw, n - 1) * n + min(cl, n - 1))); \
-            }                                                                                   \
+    {                                                                                           \
+        const bool ok_ = (R) < nt;                                                              \
+        const int rb_ = (R) <= C ? C : (R), cb_ = (R) <= C ? (R) : C;                           \
+        _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                         \
+            const int e = tid + TRI_NT * u, rw = 64 * rb_ + (e >> 6), cl = 64 * cb_ + (e & 63); \
+            const size_t o_ = ok_ ? (size_t)min(rw, n - 1) * n + min(cl, n - 1) : 0;            \
+            ra[S][u] = ld_dev((const double *)(gA + o_));                                       \
         }                                                                                       \
         _Pragma("unroll") for (int u = 0; u < 2; ++u) {                                         \
             const int e = tid + TRI_NT * u;                                                     \
-            rv[S][u] = ld_dev(Vb + (size_t)(e >> 6) * npad + 64 * (R) + (e & 63));               \
+            rv[S][u] = ld_dev(Vb + (ok_ ? (size_t)(e >> 6) * npad + 64 * (R) + (e & 63) : 0));   \
         }                                                                                       \
     }
 #define SY_PUT(S, BUF, R)                                                                       \
     {                                                                                           \
         const bool tr_ = (R) > C;                                                               \
+        const int rb_ = tr_ ? (R) : C, cb_ = tr_ ? C : (R);                                     \
         _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                         \
-            const int e = tid + TRI_NT * u;                                                     \
-            lds[SB_LA + (BUF) * 64 * 65 + (tr_ ? (e & 63) * 65 + (e >> 6) : (e >> 6) * 65 + (e & 63))] = ra[S][u]; \
+            const int e = tid + TRI_NT * u, rw = 64 * rb_ + (e >> 6), cl = 64 * cb_ + (e & 63); \
+            const double f = (rw < n && cl < n) ? 1.0 : 0.0;                                    \
+            lds[SB_LA + (BUF) * 64 * 65 + (tr_ ? (e & 63) * 65 + (e >> 6) : (e >> 6) * 65 + (e & 63))] = f * ra[S][u]; \
         }                                                                                       \
         _Pragma("unroll") for (int u = 0; u < 2; ++u) {                                         \
             const int e = tid + TRI_NT * u;                                                     \
             lds[SB_LV + (BUF) * 32 * 65 + (e >> 6) * 65 + (e & 63)] = rv[S][u];                 \
         }                                                                                       \
     }
-#define SY_STEP(J)                                                                              \
-    if (i0_ + (J) < cnt) {                                                                      \
+#define SY_STEP(J, G)                                                                           \
+    if (G) {                                                                                    \
         const int R = Bt + i0_ + (J), buf = (i0_ + (J)) & 1;                                    \
-        if (R + 1 < nt) SY_PUT(((J) + 1) & 3, buf ^ 1, R + 1)                                   \
+        SY_PUT(((J) + 1) & 3, buf ^ 1, R + 1)                                                   \
         SY_LOAD(((J) + 3) & 3, R + 3)                                                           \
         _Pragma("unroll") for (int s = 8 * h; s < 8 * h + 8; ++s) {                             \
             const double a = lds[SB_LA + buf * 64 * 65 + (16 * bi + (lane & 15)) * 65 + 4 * s + (lane >> 4)]; \
@@ -1531,7 +1543,10 @@ __device__ __noinline__ void sbr_symm_row(const double *A_, int n_, int npad_, i
 // these tiles transposed).  The row
 // operands -[V W] of block C are staged once (k-major); per tile R the
 // column operands [W V] go through a double LDS buffer and the old tile is
-// the MFMA accumulator of wavefront w's 16 x 16 block (w / 4, w % 4).
+// the MFMA accumulator of wavefront w's 16 x 16 block (w / 4, w % 4).  Old
+// entries outside the matrix are loaded from a clamped address and their
+// outputs are not stored; the ring holds the loaded values unchanged, so the
+// loads stay in flight until the accumulator takes them.
 __device__ __noinline__ void sbr_syr2k_row(double *A_, int n_, int npad_, int nt_, int C_, int Bt_, int c0_,
                                            const double *Vb_, const double *Wb_) {
     extern __shared__ double lds[];
@@ -1544,36 +1559,39 @@ __device__ __noinline__ void sbr_syr2k_row(double *A_, int n_, int npad_, int nt
     const int cnt = C - Bt + 1;  // tiles R = Bt .. C (the lower triangle)
     double *const Ls = lds + SB_LS;
     __syncthreads();
+    {  // all loads first, then the stores (one wait instead of one per load)
+        double ls[2 * SBR_B * 64 / TRI_NT];
 #pragma unroll
-    for (int u = 0; u < 2 * SBR_B * 64 / TRI_NT; ++u) {
-        const int e = tid + TRI_NT * u, k = e >> 6, i = e & 63;
-        const int kk = k < SBR_B ? k : k - SBR_B;
-        const int rw = 64 * C + i;
-        Ls[e] = -(k < SBR_B ? ld_dev(Vb + (size_t)kk * npad + rw) : ld_dev(Wb + (size_t)kk * npad + rw));
+        for (int u = 0; u < 2 * SBR_B * 64 / TRI_NT; ++u) {
+            const int e = tid + TRI_NT * u, k = e >> 6, i = e & 63;
+            const int kk = k < SBR_B ? k : k - SBR_B;
+            ls[u] = ld_dev((k < SBR_B ? Vb : Wb) + (size_t)kk * npad + 64 * C + i);
+        }
+#pragma unroll
+        for (int u = 0; u < 2 * SBR_B * 64 / TRI_NT; ++u) Ls[tid + TRI_NT * u] = -ls[u];
     }
     double ro[4][4], rr[4][4];
 #define SR_LOAD(S, R)                                                                           \
-    if ((R) < nt) {                                                                             \
+    {                                                                                           \
+        const bool ok_ = (R) <= C;                                                              \
         const int col = 64 * (R) + 16 * bj + (lane & 15);                                       \
-        _Pragma("unroll") for (int v = 0; v < 4; ++v) {                                         \
-            const double f = (row0 + 4 * v < n && col < n) ? 1.0 : 0.0;                         \
-            ro[S][v] = f * gA[(size_t)min(row0 + 4 * v, n - 1) * n + min(col, n - 1)];          \
-        }                                                                                       \
+        _Pragma("unroll") for (int v = 0; v < 4; ++v)                                          \
+            ro[S][v] = gA[ok_ ? (size_t)min(row0 + 4 * v, n - 1) * n + min(col, n - 1) : 0];    \
         _Pragma("unroll") for (int u = 0; u < 4; ++u) {                                         \
             const int e = tid + TRI_NT * u, k = e >> 6, i = e & 63;                             \
             const int kk = k < SBR_B ? k : k - SBR_B;                                           \
             const int c = 64 * (R) + i;                                                         \
-            rr[S][u] = k < SBR_B ? ld_dev(Wb + (size_t)kk * npad + c) : ld_dev(Vb + (size_t)kk * npad + c); \
+            rr[S][u] = ld_dev((k < SBR_B ? Wb : Vb) + (ok_ ? (size_t)kk * npad + c : 0));        \
         }                                                                                       \
     }
 #define SR_PUT(S, BUF)                                                                          \
     {                                                                                           \
         _Pragma("unroll") for (int u = 0; u < 4; ++u) lds[SB_RS + (BUF) * 64 * 64 + tid + TRI_NT * u] = rr[S][u]; \
     }
-#define SR_STEP(J)                                                                              \
-    if (i0_ + (J) < cnt) {                                                                      \
+#define SR_STEP(J, G)                                                                           \
+    if (G) {                                                                                    \
         const int R = Bt + i0_ + (J), buf = (i0_ + (J)) & 1;                                    \
-        if (R + 1 < nt) SR_PUT(((J) + 1) & 3, buf ^ 1)                                          \
+        SR_PUT(((J) + 1) & 3, buf ^ 1)                                                          \
         SR_LOAD(((J) + 3) & 3, R + 3)                                                           \
         blk_d4 acc;                                                                             \
         _Pragma("unroll") for (int v = 0; v < 4; ++v) acc[v] = ro[(J) & 3][v];                  \
