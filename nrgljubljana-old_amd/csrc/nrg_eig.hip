@@ -2459,10 +2459,11 @@ __global__ __launch_bounds__(64) void k_sbr_bt(const EigMat *mats, const BtItem 
     double z[WIN], zn[PF ? SBR_B : 1];
     // reflectors H(s, k), s in [s0, s1) (row n-1-s, columns k b ..; tau in
     // place of the unit first element) into LDS buffer buf
-    // (loads issued in batches of RB before their LDS stores: a load and
-    // its store in one loop iteration wait for each other, one round trip
-    // per entry)
-    constexpr int RPL = BL * SBR_B / 64, RB = RPL / 2;
+    // (all RPL loads issued before their LDS stores, so the staging waits
+    // for one round trip; a load and its store in one loop iteration wait
+    // for each other, one round trip per entry, and two batches of RPL / 2
+    // measured 1.5 % slower)
+    constexpr int RPL = BL * SBR_B / 64, RB = RPL;
     static_assert(RPL % RB == 0, "reflector staging batches");
     auto stage_refl = [&](int s0, int s1, int k, int buf) {
         const int base = s0 + 1 + k * SBR_B;
