@@ -1420,6 +1420,8 @@ constexpr int SB_PS = SB_CO + 128;            // partial-sum chunks (32 x 33)
 constexpr int SB_TOT = SB_PS + 32 * 33;          // their totals
 constexpr int SB_END = SB_TOT + 64;  // (totals, then the pivot row)
 constexpr int SB_LS = 0, SB_RS = 64 * 64;     // rank-2k operands: rows (64 x 64), columns (2 x 64 x 64), k-major
+constexpr int SB_P = 0;  // panel QR: the rows of its wavefronts, [wavefront][column][lane]
+static_assert(SB_P + SBR_MAXQ * SBR_B * 64 <= SB_CO, "the QR's panel rows leave its coefficients and sums alone");
 static_assert(SB_RS + 2 * 64 * 64 <= SB_T, "rank-2k operands overlap only the product's buffers");
 static_assert(sizeof(double) * SB_END <= TRI_COOP_LDS, "stage 1 fits the team launch's LDS");
 
@@ -1720,34 +1722,38 @@ __device__ __noinline__ int sbr_qr(SbrCtx *sc, int j0, int phase) {
         const int kmax = min(SBR_B, m - 1);
         const int nq = min(sbr_nowned(Bt, nt, rank, T), SBR_MAXQ);
         // ---- QR of the panel: wavefront q < nq holds rows of the q-th owned
-        // block (lane = row), the panel's 32 columns in registers
+        // block (lane = row), the panel's 32 columns in LDS (P[i * 64] =
+        // column i of the lane's row; in registers, beside the block sums,
+        // they spilled inside the reflector loop)
         const bool qw = w < nq;
         const int myB = qw ? sbr_qblock(Bt, nt, rank, T, w) : 0;
         const int r = 64 * myB + lane;
         const bool rok = qw && r >= c0 && r < n;
-        double p[SBR_B];
+        double *const P = lds + SB_P + (size_t)(qw ? w : 0) * SBR_B * 64 + lane;
+        if (qw) {
 #pragma unroll
-        for (int i = 0; i < SBR_B; ++i) p[i] = rok ? gA[(size_t)r * n + j0 + i] : 0.0;
+            for (int i0 = 0; i0 < SBR_B; i0 += 16) {
+                double t[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) t[i] = gA[(size_t)min(r, n - 1) * n + j0 + i0 + i];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) P[(i0 + i) * 64] = rok ? t[i] : 0.0;
+            }
+        }
         for (int k = 0; k < kmax; ++k) {
             const int par = k & 1, rp = c0 + k;
-            double xk = 0.0;
-#pragma unroll
-            for (int i = 0; i < SBR_B; ++i) xk = i == k ? p[i] : xk;
+            const double xk = qw ? P[k * 64] : 0.0;
             if (qw) {
                 const bool below = rok && r > rp;
                 const double sg = wave_sum(below ? xk * xk : 0.0);
                 if (lane == 0) st_dev(part + ((size_t)par * nt + myB) * SBR_B, sg);
-#pragma unroll
-                for (int i = 1; i < SBR_B; ++i) {
-                    if (i <= k) continue;
-                    const double d = wave_sum(below ? xk * p[i] : 0.0);
+#pragma unroll 2
+                for (int i = k + 1; i < SBR_B; ++i) {
+                    const double d = wave_sum(below ? xk * P[i * 64] : 0.0);
                     if (lane == 0) st_dev(part + ((size_t)par * nt + myB) * SBR_B + i, d);
                 }
-                if (r == rp) {
-#pragma unroll
-                    for (int i = 0; i < SBR_B; ++i)
-                        if (i >= k) st_dev(piv + par * SBR_B + i, p[i]);
-                }
+                if (r == rp)
+                    for (int i = k; i < SBR_B; ++i) st_dev(piv + par * SBR_B + i, P[i * 64]);
             }
             SB_PROF(1);
             ++phase;
@@ -1771,6 +1777,7 @@ __device__ __noinline__ int sbr_qr(SbrCtx *sc, int j0, int phase) {
                 if (g == 0) lds[SB_TOT + 32 + i] = pv;
                 __syncthreads();
             }
+            SB_PROF(11);
             if (tid < 32) {
                 double tot = 0.0;
                 for (int gg = 0; gg < 32; ++gg) tot += lds[SB_PS + gg * 33 + tid];
@@ -1798,11 +1805,9 @@ __device__ __noinline__ int sbr_qr(SbrCtx *sc, int j0, int phase) {
             if (qw) {
                 const double scale = co[34], beta = co[33];
                 const double v = r > rp ? xk * scale : 1.0;
-#pragma unroll
-                for (int i = 0; i < SBR_B; ++i) {
-                    if (i == k) p[i] = r > rp ? v : r == rp ? beta : p[i];
-                    else if (i > k && (r >= rp) && rok) p[i] -= co[i] * v;
-                }
+                P[k * 64] = r > rp ? v : r == rp ? beta : xk;
+                if (r >= rp && rok)
+                    for (int i = k + 1; i < SBR_B; ++i) P[i * 64] -= co[i] * v;
             }
             // (co and the partial-sum table are next written after the next
             // team barrier, which every wavefront reaches after this update)
@@ -1819,8 +1824,9 @@ __device__ __noinline__ int sbr_qr(SbrCtx *sc, int j0, int phase) {
             double *const P2 = ws + 2 * (size_t)n * n;
 #pragma unroll
             for (int i = 0; i < SBR_B; ++i) {
-                if (rok) P2[(size_t)(j0 + i) * n + r] = p[i];
-                const double v = (i < kmax && rok) ? (r > c0 + i ? p[i] : r == c0 + i ? 1.0 : 0.0) : 0.0;
+                const double pi = P[i * 64];
+                if (rok) P2[(size_t)(j0 + i) * n + r] = pi;
+                const double v = (i < kmax && rok) ? (r > c0 + i ? pi : r == c0 + i ? 1.0 : 0.0) : 0.0;
                 st_dev(Vb + (size_t)i * npad + r, v);
             }
         }
@@ -1919,21 +1925,19 @@ __device__ __noinline__ int sbr_upd(SbrCtx *sc, int j0, int phase) {
         }
         __syncthreads();
         __syncthreads();
-        // T in one wavefront: lane i < 32 keeps row i in registers (T is
-        // upper triangular, T[i][k] = -tau_k sum_{l<k} T[i][l] G[l][k])
-        if (w == 0) {
-            double trow[SBR_B];
-#pragma unroll
+        // T in one wavefront: lane i < 32 builds row i in LDS, column by
+        // column (T is upper triangular, T[i][k] = -tau_k sum_{l<k} T[i][l]
+        // G[l][k]); a row held in registers through the unrolled triangle
+        // spilled
+        if (w == 0 && lane < SBR_B) {
+            double *const Trow = lds + SB_T + lane * 33;
+#pragma unroll 1
             for (int k = 0; k < SBR_B; ++k) {
                 const double tk = co[64 + k];
                 double t = 0.0;
-#pragma unroll
-                for (int l = 0; l < k; ++l) t = fma(trow[l], lds[SB_G + l * 33 + k], t);
-                trow[k] = lane < k ? -tk * t : lane == k ? tk : 0.0;
-            }
-            if (lane < SBR_B) {
-#pragma unroll
-                for (int k = 0; k < SBR_B; ++k) lds[SB_T + lane * 33 + k] = trow[k];
+#pragma unroll 4
+                for (int l = 0; l < k; ++l) t = fma(Trow[l], lds[SB_G + l * 33 + k], t);
+                Trow[k] = lane < k ? -tk * t : lane == k ? tk : 0.0;
             }
         }
         __syncthreads();
@@ -1970,6 +1974,9 @@ __device__ __noinline__ int sbr_upd(SbrCtx *sc, int j0, int phase) {
                 const int e = tid + TRI_NT * u, jj = e >> 6, rr = e & 63;  // W[64 C + rr][jj]
                 const int rw = 64 * C + rr;
                 double t1 = 0.0, t2 = 0.0;
+                // (rolled: fully unrolled, the T and S reads were hoisted
+                // out of the block loop into registers, which spilled)
+#pragma unroll 4
                 for (int l = 0; l < SBR_B; ++l) {
                     t1 = fma(Ly[rr * 33 + l], lds[SB_T + l * 33 + jj], t1);
                     t2 = fma(Lv[rr * 33 + l], lds[SB_S + l * 33 + jj], t2);
@@ -2038,6 +2045,7 @@ __device__ __noinline__ void sbr_blk(const EigMat &mt, int rank, int T, int *bar
     if (rank == 0 && tid == 0)
         printf("sbrprof n=%d T=%d mirror %lld qr %lld qrbar %lld qrupd %lld wb+bar %lld symm %lld bar %lld gram+bar %lld tsw %lld bar %lld syr2k %lld (x10ns)\n",
                n, T, sc.pacc[0], sc.pacc[1], sc.pacc[2], sc.pacc[3], sc.pacc[4], sc.pacc[5], sc.pacc[6], sc.pacc[7], sc.pacc[8], sc.pacc[9], sc.pacc[10]);
+    if (rank == 0 && tid == 0) printf("sbrprof2 n=%d T=%d bsum %lld\n", n, T, sc.pacc[11]);
 #endif
 }
 #undef SB_PROF
