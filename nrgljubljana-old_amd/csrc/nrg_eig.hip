@@ -188,6 +188,7 @@ struct TriJob {
 // instructions: a scalar base plus a 32-bit lane offset instead of a 64-bit
 // address per element)
 typedef __attribute__((address_space(1))) double gdouble;
+typedef __attribute__((address_space(1))) int gint;
 __device__ inline void st_dev(double *p, double v) {
     __hip_atomic_store((gdouble *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -2171,7 +2172,7 @@ __global__ __launch_bounds__(256) void k_sbr_band(const EigMat *mats, const BtIt
     }
 }
 
-__device__ __forceinline__ bool ch_wait(const int *prog, int need, int *status, int *s_flag) {
+__device__ __forceinline__ bool ch_wait(const gint *prog, int need, int *status, int *s_flag) {
     if (threadIdx.x == 0) {
         long long spins = 0;
         int ok = 1;
@@ -2194,8 +2195,12 @@ __device__ __forceinline__ bool ch_sweep(const EigMat &mt, int s, double (*ring)
                                          double *wq, double *db, double *sc, int *status, int *s_flag) {
     const int n = mt.n, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     double *const band = sbr_band(mt.ws, n);
-    int *const prog = sbr_prog(mt.ws, n);
-    double *const gdg = vslot(mt.ws, n, V_DG), *const god = vslot(mt.ws, n, V_OD);
+    // (global-address pointers: a generic store also counts against the LDS
+    // counter and may alias LDS, so the task's next LDS wait or barrier
+    // waited for the store to reach memory)
+    gint *const prog = (gint *)sbr_prog(mt.ws, n);
+    gdouble *const gdg = (gdouble *)vslot(mt.ws, n, V_DG), *const god = (gdouble *)vslot(mt.ws, n, V_OD);
+    gdouble *const gA = (gdouble *)mt.A;
     auto slot = [&](int j) { return (j - s - 1 + CH_W) % CH_W; };
     auto L = [&](int j, int d) -> double & { return ring[slot(j)][d]; };
     // (at most b + 1 columns: the loads of a thread are issued together,
@@ -2254,7 +2259,7 @@ __device__ __forceinline__ bool ch_sweep(const EigMat &mt, int s, double (*ring)
             if (lane < len) {
                 L(c, r0 - c + lane) = lane == 0 ? beta : 0.0;
                 // reflector (s, k): row n - 1 - s, columns k b ..
-                mt.A[(size_t)(n - 1 - s) * n + k * SBR_B + lane] = lane == 0 ? tau : x * scale;
+                gA[(size_t)(n - 1 - s) * n + k * SBR_B + lane] = lane == 0 ? tau : x * scale;
             }
             if (lane == 0) {
                 sc[0] = tau;
