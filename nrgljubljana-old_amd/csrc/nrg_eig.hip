@@ -3688,12 +3688,14 @@ __global__ __launch_bounds__(BT_NT) void k_eig_bt(const EigMat *mats, const BtIt
 //     lives in S1; G = Y^T Y, T, M and W = Z M^T in S2.
 // ---------------------------------------------------------------------------
 constexpr int WY_NB = 128;
+constexpr int WY_BB = 2 * WY_NB;  // reflectors per compact-WY block: T from its two halves
 constexpr int BT_WY_MIN = 1025;   // register kernel below, WY from here
 constexpr int EIG_MAX_N = 32768;
 
 struct WyItem {
     double *ws;
     int n, j0, nb, pad;
+    int go;  // T factor of one half: offset of the half in its block's G and T
 };
 
 // Y^T: row j = reflector j (columns j+1.., first element included), zero
@@ -3732,8 +3734,9 @@ __global__ __launch_bounds__(WY_TF_NT) void k_wy_tfactor(const WyItem *items) {
     const WyItem it = items[blockIdx.x];
     const int n = it.n, nb = it.nb, tid = threadIdx.x, i = tid >> 2, q = tid & 3;
     double *S2 = it.ws + 2 * (size_t)n * n;
-    const double *G = S2;
-    double *T = S2 + WY_NB * WY_NB;
+    // this half's diagonal blocks of the block's G and T (WY_BB x WY_BB)
+    const double *G = S2 + (size_t)it.go * (WY_BB + 1);
+    double *T = S2 + (size_t)WY_BB * WY_BB + (size_t)it.go * (WY_BB + 1);
     const double *tau = vslot(it.ws, n, V_TAU) + it.j0;
     // G's strict upper triangle, column k from row 0: loads issued before
     // the LDS stores (a load and its store in one iteration serialise)
@@ -3748,7 +3751,7 @@ __global__ __launch_bounds__(WY_TF_NT) void k_wy_tfactor(const WyItem *items) {
             while (k * (k - 1) / 2 > e) --k;
             while ((k + 1) * k / 2 <= e) ++k;
             const int l = e - k * (k - 1) / 2;
-            const double v = G[(size_t)min(l, WY_NB - 1) * WY_NB + min(k, WY_NB - 1)];
+            const double v = G[(size_t)min(l, WY_NB - 1) * WY_BB + min(k, WY_NB - 1)];
             g[u] = (e < WY_TRI && k < nb) ? v : 0.0;
         }
 #pragma unroll
@@ -3777,8 +3780,11 @@ __global__ __launch_bounds__(WY_TF_NT) void k_wy_tfactor(const WyItem *items) {
             if (i == k) sT[ri + k] = tk;
         }
     }
-    if (i < nb)
-        for (int k = q; k < nb; k += 4) T[i * WY_NB + k] = k >= i ? sT[ri + k] : 0.0;
+    if (i < nb) {
+        for (int k = q; k < nb; k += 4) T[i * WY_BB + k] = k >= i ? sT[ri + k] : 0.0;
+        // the second half's rows of T are zero left of its diagonal block
+        for (int k = q; k < it.go; k += 4) T[i * WY_BB + k - it.go] = 0.0;
+    }
 }
 
 // Row c of Z (the back-transformed eigenvector c) -> row rank(c) of A.
@@ -3937,10 +3943,10 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             continue;
         }
         // (S2 also holds the compact-WY scratch of the back-transformation:
-        // G and T of a block, M = T Y^T and W, 2 WY_NB (WY_NB + n) doubles)
+        // G and T of a block, M = T Y^T and W, 2 WY_BB (WY_BB + n) doubles)
         if (tasks[i].n >= blk_min_n && c->sbr && sbr_ws_doubles(tasks[i].n) <= (size_t)tasks[i].n * tasks[i].n &&
             (size_t)tasks[i].n * (CH_W + 1) + 64 <= (size_t)tasks[i].n * tasks[i].n &&
-            2 * (size_t)WY_NB * (WY_NB + tasks[i].n) <= (size_t)tasks[i].n * tasks[i].n) {
+            2 * (size_t)WY_BB * (WY_BB + tasks[i].n) <= (size_t)tasks[i].n * tasks[i].n) {
             blkjob[i] = 2;  // two-stage; team size below, once all blocked jobs are known
             mats[i].pad = 2;
             continue;
@@ -4383,11 +4389,11 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             for (int c0 = 0; c0 < n; c0 += 64) sbt.push_back({i, c0});
             for (int t = 0; t < ns * ns; ++t) str.push_back({i, t});
             wymat.push_back(i);
-            wy_blocks = std::max(wy_blocks, (sbr_nrefl(n) + WY_NB - 1) / WY_NB);
+            wy_blocks = std::max(wy_blocks, (sbr_nrefl(n) + WY_BB - 1) / WY_BB);
             for (int r = 0; r < n; ++r) wyrows.push_back({i, r});
         } else if (n >= BT_WY_MIN) {
             wymat.push_back(i);
-            wy_blocks = std::max(wy_blocks, (n - 2 + WY_NB - 1) / WY_NB);
+            wy_blocks = std::max(wy_blocks, (n - 2 + WY_BB - 1) / WY_BB);
             for (int r = 0; r < n; ++r) wyrows.push_back({i, r});
         } else {
             int v = 0;
@@ -4399,68 +4405,79 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         for (int c0 = 0; c0 < n; c0 += 64) rk.push_back({i, c0});
     }
     // WY blocks, counted from the last: block q of matrix i covers reflectors
-    // [max(0, j1 - WY_NB), j1) with j1 = NR - q WY_NB: NR = n - 2 reflectors
+    // [max(0, j1 - WY_BB), j1) with j1 = NR - q WY_BB: NR = n - 2 reflectors
     // starting at row j + 1 (one-stage), or sbr_nrefl(n) starting at row
-    // j + SBR_B (two-stage; WyItem.pad holds that offset)
-    std::vector<WyItem> wyitems;
+    // j + SBR_B (two-stage; WyItem.pad holds that offset).  T of a block
+    // (dlarft's block recurrence): T = [T1 T12; 0 T2] with T1, T2 of its
+    // halves (k_wy_tfactor) and T12 = -T1 (Y1^T Y2) T2 (two GEMM stages).
+    std::vector<WyItem> wyitems, wyblk;  // T-factor halves; whole blocks
     std::vector<nrg_gemm_out> wyouts;
     std::vector<nrg_gemm_term> wyterms;
     std::vector<GemmItem> wygi, wyred;
     std::vector<int> wyks;
+    constexpr int WY_STAGES = 6;  // G, P = G12 T2, T12 = -T1 P, M, W, Z
     struct WyLevel {
         int w0, w1;
-        int g[4][2];  // gemm item ranges of G, M, W, Z
-        int r[4][2];  // reduce item ranges
-        bool mfma[4];
+        int g[WY_STAGES][2];  // gemm item ranges per stage
+        int r[WY_STAGES][2];  // reduce item ranges
+        bool mfma[WY_STAGES];
         double flops;
     };
     std::vector<WyLevel> wylev(wy_blocks);
     for (int q = 0; q < wy_blocks; ++q) {
         WyLevel L{};
         L.w0 = (int)wyitems.size();
-        std::vector<int> act;
+        const size_t b0 = wyblk.size();
         for (int i : wymat) {
             const int n = tasks[i].n, two = mats[i].pad == 2;
-            const int j1 = (two ? sbr_nrefl(n) : n - 2) - q * WY_NB;
+            const int j1 = (two ? sbr_nrefl(n) : n - 2) - q * WY_BB;
             if (j1 <= 0) continue;
-            const int j0 = std::max(0, j1 - WY_NB);
-            wyitems.push_back({c->ews + off[i], n, j0, j1 - j0, two ? SBR_B : 1});
-            act.push_back(i);
+            const int j0 = std::max(0, j1 - WY_BB), nb = j1 - j0, nb1 = std::min(nb, WY_NB);
+            wyblk.push_back({c->ews + off[i], n, j0, nb, two ? SBR_B : 1, 0});
+            wyitems.push_back({c->ews + off[i], n, j0, nb1, two ? SBR_B : 1, 0});
+            if (nb > nb1) wyitems.push_back({c->ews + off[i], n, j0 + nb1, nb - nb1, two ? SBR_B : 1, WY_NB});
         }
         L.w1 = (int)wyitems.size();
-        for (int stage = 0; stage < 4; ++stage) {
+        for (int stage = 0; stage < WY_STAGES; ++stage) {
             L.g[stage][0] = (int)wygi.size();
             const size_t so0 = wyouts.size();
-            for (int w = L.w0; w < L.w1; ++w) {
-                const WyItem &it = wyitems[w];
+            for (size_t w = b0; w < wyblk.size(); ++w) {
+                const WyItem &it = wyblk[w];
                 const int n = it.n, j0 = it.j0, nb = it.nb, ro = it.pad, np = n - j0 - ro;
+                const int nb1 = std::min(nb, WY_NB), nb2 = nb - nb1;
+                if ((stage == 1 || stage == 2) && nb2 == 0) continue;  // one half: T = T1
                 double *ws = it.ws, *Yt = ws + (size_t)n * n, *S2 = ws + 2 * (size_t)n * n;
-                double *G = S2, *T = S2 + WY_NB * WY_NB, *M = S2 + 2 * WY_NB * WY_NB, *W = M + (size_t)WY_NB * n;
+                double *G = S2, *T = S2 + WY_BB * WY_BB, *M = S2 + 2 * WY_BB * WY_BB, *W = M + (size_t)WY_BB * n;
+                double *P = W;  // G12 T2 (WY_NB x WY_NB), before W is computed
                 const double *Yb = Yt + (size_t)j0 * n + j0 + ro;  // block rows, columns j0+ro..
                 nrg_gemm_out o{};
                 nrg_gemm_term t{};
                 t.alpha = 1.0;
                 if (stage == 0) {  // G = Yb Yb^T
-                    o.C = G; o.m = nb; o.n = nb; o.ldc = WY_NB;
+                    o.C = G; o.m = nb; o.n = nb; o.ldc = WY_BB;
                     t.A = Yb; t.lda = n; t.B = Yb; t.ldb = n; t.transB = 1; t.k = np;
-                } else if (stage == 1) {  // M = T Yb
+                } else if (stage == 1) {  // P = G12 T2
+                    o.C = P; o.m = nb1; o.n = nb2; o.ldc = WY_NB;
+                    t.A = G + nb1; t.lda = WY_BB; t.B = T + (size_t)nb1 * WY_BB + nb1; t.ldb = WY_BB; t.k = nb2;
+                } else if (stage == 2) {  // T12 = -T1 P
+                    o.C = T + nb1; o.m = nb1; o.n = nb2; o.ldc = WY_BB;
+                    t.A = T; t.lda = WY_BB; t.B = P; t.ldb = WY_NB; t.k = nb1; t.alpha = -1.0;
+                } else if (stage == 3) {  // M = T Yb
                     o.C = M; o.m = nb; o.n = np; o.ldc = n;
-                    t.A = T; t.lda = WY_NB; t.B = Yb; t.ldb = n; t.k = nb;
-                } else if (stage == 2) {  // W = Z[:, j0+ro:] M^T
-                    o.C = W; o.m = n; o.n = nb; o.ldc = WY_NB;
+                    t.A = T; t.lda = WY_BB; t.B = Yb; t.ldb = n; t.k = nb;
+                } else if (stage == 4) {  // W = Z[:, j0+ro:] M^T
+                    o.C = W; o.m = n; o.n = nb; o.ldc = WY_BB;
                     t.A = ws + j0 + ro; t.lda = n; t.B = M; t.ldb = n; t.transB = 1; t.k = np;
                 } else {  // Z[:, j0+ro:] -= W Yb
                     o.C = ws + j0 + ro; o.m = n; o.n = np; o.ldc = n; o.beta = 1.0;
-                    t.A = W; t.lda = WY_NB; t.B = Yb; t.ldb = n; t.k = nb; t.alpha = -1.0;
+                    t.A = W; t.lda = WY_BB; t.B = Yb; t.ldb = n; t.k = nb; t.alpha = -1.0;
                 }
                 o.term_begin = (int)wyterms.size();
                 o.term_count = 1;
-                const int g = (int)wyouts.size();
                 wyouts.push_back(o);
                 wyterms.push_back(t);
                 wyks.push_back((t.k + GK - 1) / GK);
-                (void)g;
-                if (stage >= 2) L.flops += 2.0 * n * (double)nb * np;
+                if (stage >= 4) L.flops += 2.0 * n * (double)nb * np;
             }
             L.r[stage][0] = (int)wyred.size();
             L.mfma[stage] = gemm_plan(wyouts, so0, wyks, wygi, wyred, gslots, c->gemm_mfma_min_tiles);
@@ -4657,12 +4674,10 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
                 hipLaunchKernelGGL(k_wy_tfactor, dim3(L.w1 - L.w0), dim3(WY_TF_NT), 0, c->stream, d_wyitems + L.w0);
             }
             HIPCHK(c, hipGetLastError());
-            s = wy_gemm(L, 1, 0.0);
-            if (s) return s;
-            s = wy_gemm(L, 2, 0.5 * L.flops);
-            if (s) return s;
-            s = wy_gemm(L, 3, 0.5 * L.flops);
-            if (s) return s;
+            for (int stage = 1; stage < WY_STAGES; ++stage) {
+                s = wy_gemm(L, stage, stage >= 4 ? 0.5 * L.flops : 0.0);
+                if (s) return s;
+            }
         }
         {
             LaunchTimer lt(K_EIG_COPY, c->stream, 0.0, 0.0);
