@@ -266,7 +266,9 @@ def main():
                    "f/singlet recalculation; eigenvectors kept for the FDM pass)",
                    "iterations_timed": [a.warmup, a.warmup + a.steps - 1], "eigenpairs_per_step": pairs / a.steps,
                    "kernels": {k: {"launches": v["launches"], "ms": v["ms"]} for k, v in kst.items() if v["launches"]},
-                   "profile": "kernels: HIP events on every launch of the run (warmup included)"},
+                   "profile": "kernels: HIP events on every launch of the run (warmup included)",
+                   "full_solve": "the whole solve (pass 1, FDM sweep, spectra) is timed by bin/nrg, not here: "
+                                 "profiles/full_solve_anderson2ch_keep10000_fdm.md"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain_per_block": plain,
