@@ -3688,14 +3688,21 @@ __global__ __launch_bounds__(BT_NT) void k_eig_bt(const EigMat *mats, const BtIt
 //     lives in S1; G = Y^T Y, T, M and W = Z M^T in S2.
 // ---------------------------------------------------------------------------
 constexpr int WY_NB = 128;
-constexpr int WY_BB = 2 * WY_NB;  // reflectors per compact-WY block: T from its two halves
+constexpr int WY_BB = 2 * WY_NB;  // reflectors per compact-WY block, at least: T from chunks of WY_NB
+constexpr int WY_BB_MAX = 4 * WY_NB;  // and at most, where the workspace holds its scratch
+// reflectors per WY block of a matrix of dimension n: the largest of
+// WY_BB_MAX, WY_BB whose G, T, M and W (2 B (B + n) doubles) fit S2 (n^2)
+inline int wy_bb(int n) {
+    return 2 * (size_t)WY_BB_MAX * (WY_BB_MAX + n) <= (size_t)n * n ? WY_BB_MAX : WY_BB;
+}
 constexpr int BT_WY_MIN = 1025;   // register kernel below, WY from here
 constexpr int EIG_MAX_N = 32768;
 
 struct WyItem {
     double *ws;
     int n, j0, nb, pad;
-    int go;  // T factor of one half: offset of the half in its block's G and T
+    int go;  // T factor of one chunk: offset of the chunk in its block's G and T
+    int ld;  // reflectors per block of the matrix: leading dimension of G and T
 };
 
 // Y^T: row j = reflector j (columns j+1.., first element included), zero
@@ -3734,9 +3741,10 @@ __global__ __launch_bounds__(WY_TF_NT) void k_wy_tfactor(const WyItem *items) {
     const WyItem it = items[blockIdx.x];
     const int n = it.n, nb = it.nb, tid = threadIdx.x, i = tid >> 2, q = tid & 3;
     double *S2 = it.ws + 2 * (size_t)n * n;
-    // this half's diagonal blocks of the block's G and T (WY_BB x WY_BB)
-    const double *G = S2 + (size_t)it.go * (WY_BB + 1);
-    double *T = S2 + (size_t)WY_BB * WY_BB + (size_t)it.go * (WY_BB + 1);
+    // this chunk's diagonal blocks of the block's G and T (ld x ld)
+    const int ld = it.ld;
+    const double *G = S2 + (size_t)it.go * (ld + 1);
+    double *T = S2 + (size_t)ld * ld + (size_t)it.go * (ld + 1);
     const double *tau = vslot(it.ws, n, V_TAU) + it.j0;
     // G's strict upper triangle, column k from row 0: loads issued before
     // the LDS stores (a load and its store in one iteration serialise)
@@ -3751,7 +3759,7 @@ __global__ __launch_bounds__(WY_TF_NT) void k_wy_tfactor(const WyItem *items) {
             while (k * (k - 1) / 2 > e) --k;
             while ((k + 1) * k / 2 <= e) ++k;
             const int l = e - k * (k - 1) / 2;
-            const double v = G[(size_t)min(l, WY_NB - 1) * WY_BB + min(k, WY_NB - 1)];
+            const double v = G[(size_t)min(l, WY_NB - 1) * ld + min(k, WY_NB - 1)];
             g[u] = (e < WY_TRI && k < nb) ? v : 0.0;
         }
 #pragma unroll
@@ -3781,9 +3789,9 @@ __global__ __launch_bounds__(WY_TF_NT) void k_wy_tfactor(const WyItem *items) {
         }
     }
     if (i < nb) {
-        for (int k = q; k < nb; k += 4) T[i * WY_BB + k] = k >= i ? sT[ri + k] : 0.0;
-        // the second half's rows of T are zero left of its diagonal block
-        for (int k = q; k < it.go; k += 4) T[i * WY_BB + k - it.go] = 0.0;
+        for (int k = q; k < nb; k += 4) T[(size_t)i * ld + k] = k >= i ? sT[ri + k] : 0.0;
+        // a later chunk's rows of T are zero left of its diagonal block
+        for (int k = q; k < it.go; k += 4) T[(size_t)i * ld + k - it.go] = 0.0;
     }
 }
 
@@ -4389,11 +4397,11 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             for (int c0 = 0; c0 < n; c0 += 64) sbt.push_back({i, c0});
             for (int t = 0; t < ns * ns; ++t) str.push_back({i, t});
             wymat.push_back(i);
-            wy_blocks = std::max(wy_blocks, (sbr_nrefl(n) + WY_BB - 1) / WY_BB);
+            wy_blocks = std::max(wy_blocks, (sbr_nrefl(n) + wy_bb(n) - 1) / wy_bb(n));
             for (int r = 0; r < n; ++r) wyrows.push_back({i, r});
         } else if (n >= BT_WY_MIN) {
             wymat.push_back(i);
-            wy_blocks = std::max(wy_blocks, (n - 2 + WY_BB - 1) / WY_BB);
+            wy_blocks = std::max(wy_blocks, (n - 2 + wy_bb(n) - 1) / wy_bb(n));
             for (int r = 0; r < n; ++r) wyrows.push_back({i, r});
         } else {
             int v = 0;
@@ -4405,17 +4413,20 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         for (int c0 = 0; c0 < n; c0 += 64) rk.push_back({i, c0});
     }
     // WY blocks, counted from the last: block q of matrix i covers reflectors
-    // [max(0, j1 - WY_BB), j1) with j1 = NR - q WY_BB: NR = n - 2 reflectors
-    // starting at row j + 1 (one-stage), or sbr_nrefl(n) starting at row
-    // j + SBR_B (two-stage; WyItem.pad holds that offset).  T of a block
-    // (dlarft's block recurrence): T = [T1 T12; 0 T2] with T1, T2 of its
-    // halves (k_wy_tfactor) and T12 = -T1 (Y1^T Y2) T2 (two GEMM stages).
-    std::vector<WyItem> wyitems, wyblk;  // T-factor halves; whole blocks
+    // [max(0, j1 - B), j1) with j1 = NR - q B, B = wy_bb(n): NR = n - 2
+    // reflectors starting at row j + 1 (one-stage), or sbr_nrefl(n) starting
+    // at row j + SBR_B (two-stage; WyItem.pad holds that offset).  T of a
+    // block by dlarft's block recurrence over chunks of WY_NB: the diagonal
+    // blocks by k_wy_tfactor, then chunk c's column above them,
+    // T[0:c, c] = -T[0:c, 0:c] (Y[0:c]^T Y_c) T_cc, by two GEMM stages (P_c =
+    // G[0:c, c] T_cc, then the product with the part of T already built).
+    std::vector<WyItem> wyitems, wyblk;  // T-factor chunks; whole blocks
     std::vector<nrg_gemm_out> wyouts;
     std::vector<nrg_gemm_term> wyterms;
     std::vector<GemmItem> wygi, wyred;
     std::vector<int> wyks;
-    constexpr int WY_STAGES = 6;  // G, P = G12 T2, T12 = -T1 P, M, W, Z
+    constexpr int WY_NCH = WY_BB_MAX / WY_NB;
+    constexpr int WY_STAGES = 2 * WY_NCH + 2;  // G, (P_c, T_c) for c = 1 .., M, W, Z
     struct WyLevel {
         int w0, w1;
         int g[WY_STAGES][2];  // gemm item ranges per stage
@@ -4429,13 +4440,13 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
         L.w0 = (int)wyitems.size();
         const size_t b0 = wyblk.size();
         for (int i : wymat) {
-            const int n = tasks[i].n, two = mats[i].pad == 2;
-            const int j1 = (two ? sbr_nrefl(n) : n - 2) - q * WY_BB;
+            const int n = tasks[i].n, two = mats[i].pad == 2, bb = wy_bb(n);
+            const int j1 = (two ? sbr_nrefl(n) : n - 2) - q * bb;
             if (j1 <= 0) continue;
-            const int j0 = std::max(0, j1 - WY_BB), nb = j1 - j0, nb1 = std::min(nb, WY_NB);
-            wyblk.push_back({c->ews + off[i], n, j0, nb, two ? SBR_B : 1, 0});
-            wyitems.push_back({c->ews + off[i], n, j0, nb1, two ? SBR_B : 1, 0});
-            if (nb > nb1) wyitems.push_back({c->ews + off[i], n, j0 + nb1, nb - nb1, two ? SBR_B : 1, WY_NB});
+            const int j0 = std::max(0, j1 - bb), nb = j1 - j0;
+            wyblk.push_back({c->ews + off[i], n, j0, nb, two ? SBR_B : 1, 0, bb});
+            for (int g0 = 0; g0 < nb; g0 += WY_NB)
+                wyitems.push_back({c->ews + off[i], n, j0 + g0, std::min(WY_NB, nb - g0), two ? SBR_B : 1, g0, bb});
         }
         L.w1 = (int)wyitems.size();
         for (int stage = 0; stage < WY_STAGES; ++stage) {
@@ -4443,41 +4454,42 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             const size_t so0 = wyouts.size();
             for (size_t w = b0; w < wyblk.size(); ++w) {
                 const WyItem &it = wyblk[w];
-                const int n = it.n, j0 = it.j0, nb = it.nb, ro = it.pad, np = n - j0 - ro;
-                const int nb1 = std::min(nb, WY_NB), nb2 = nb - nb1;
-                if ((stage == 1 || stage == 2) && nb2 == 0) continue;  // one half: T = T1
+                const int n = it.n, j0 = it.j0, nb = it.nb, ro = it.pad, np = n - j0 - ro, ld = it.ld;
+                // chunk stages: chunk ch = stage / 2 + ... (stages 1, 2: chunk 1; 3, 4: chunk 2; ..)
+                const int ch = (stage + 1) / 2, c0 = ch * WY_NB, nbc = std::min(WY_NB, nb - c0);
+                if (stage >= 1 && stage < WY_STAGES - 3 && nbc <= 0) continue;
                 double *ws = it.ws, *Yt = ws + (size_t)n * n, *S2 = ws + 2 * (size_t)n * n;
-                double *G = S2, *T = S2 + WY_BB * WY_BB, *M = S2 + 2 * WY_BB * WY_BB, *W = M + (size_t)WY_BB * n;
-                double *P = W;  // G12 T2 (WY_NB x WY_NB), before W is computed
+                double *G = S2, *T = S2 + (size_t)ld * ld, *M = S2 + 2 * (size_t)ld * ld, *W = M + (size_t)ld * n;
+                double *P = W;  // P_c (c0 x nbc, ld WY_NB), before W is computed
                 const double *Yb = Yt + (size_t)j0 * n + j0 + ro;  // block rows, columns j0+ro..
                 nrg_gemm_out o{};
                 nrg_gemm_term t{};
                 t.alpha = 1.0;
                 if (stage == 0) {  // G = Yb Yb^T
-                    o.C = G; o.m = nb; o.n = nb; o.ldc = WY_BB;
+                    o.C = G; o.m = nb; o.n = nb; o.ldc = ld;
                     t.A = Yb; t.lda = n; t.B = Yb; t.ldb = n; t.transB = 1; t.k = np;
-                } else if (stage == 1) {  // P = G12 T2
-                    o.C = P; o.m = nb1; o.n = nb2; o.ldc = WY_NB;
-                    t.A = G + nb1; t.lda = WY_BB; t.B = T + (size_t)nb1 * WY_BB + nb1; t.ldb = WY_BB; t.k = nb2;
-                } else if (stage == 2) {  // T12 = -T1 P
-                    o.C = T + nb1; o.m = nb1; o.n = nb2; o.ldc = WY_BB;
-                    t.A = T; t.lda = WY_BB; t.B = P; t.ldb = WY_NB; t.k = nb1; t.alpha = -1.0;
-                } else if (stage == 3) {  // M = T Yb
+                } else if (stage < WY_STAGES - 3 && (stage & 1)) {  // P_c = G[0:c0, c0:] T_cc
+                    o.C = P; o.m = c0; o.n = nbc; o.ldc = WY_NB;
+                    t.A = G + c0; t.lda = ld; t.B = T + (size_t)c0 * ld + c0; t.ldb = ld; t.k = nbc;
+                } else if (stage < WY_STAGES - 3) {  // T[0:c0, c0:] = -T[0:c0, 0:c0] P_c
+                    o.C = T + c0; o.m = c0; o.n = nbc; o.ldc = ld;
+                    t.A = T; t.lda = ld; t.B = P; t.ldb = WY_NB; t.k = c0; t.alpha = -1.0;
+                } else if (stage == WY_STAGES - 3) {  // M = T Yb
                     o.C = M; o.m = nb; o.n = np; o.ldc = n;
-                    t.A = T; t.lda = WY_BB; t.B = Yb; t.ldb = n; t.k = nb;
-                } else if (stage == 4) {  // W = Z[:, j0+ro:] M^T
-                    o.C = W; o.m = n; o.n = nb; o.ldc = WY_BB;
+                    t.A = T; t.lda = ld; t.B = Yb; t.ldb = n; t.k = nb;
+                } else if (stage == WY_STAGES - 2) {  // W = Z[:, j0+ro:] M^T
+                    o.C = W; o.m = n; o.n = nb; o.ldc = ld;
                     t.A = ws + j0 + ro; t.lda = n; t.B = M; t.ldb = n; t.transB = 1; t.k = np;
                 } else {  // Z[:, j0+ro:] -= W Yb
                     o.C = ws + j0 + ro; o.m = n; o.n = np; o.ldc = n; o.beta = 1.0;
-                    t.A = W; t.lda = WY_BB; t.B = Yb; t.ldb = n; t.k = nb; t.alpha = -1.0;
+                    t.A = W; t.lda = ld; t.B = Yb; t.ldb = n; t.k = nb; t.alpha = -1.0;
                 }
                 o.term_begin = (int)wyterms.size();
                 o.term_count = 1;
                 wyouts.push_back(o);
                 wyterms.push_back(t);
                 wyks.push_back((t.k + GK - 1) / GK);
-                if (stage >= 4) L.flops += 2.0 * n * (double)nb * np;
+                if (stage >= WY_STAGES - 2) L.flops += 2.0 * n * (double)nb * np;
             }
             L.r[stage][0] = (int)wyred.size();
             L.mfma[stage] = gemm_plan(wyouts, so0, wyks, wygi, wyred, gslots, c->gemm_mfma_min_tiles);
@@ -4675,7 +4687,7 @@ extern "C" nrg_status nrg_diag_batch(nrg_ctx *c, const nrg_eig_task *tasks_in, i
             }
             HIPCHK(c, hipGetLastError());
             for (int stage = 1; stage < WY_STAGES; ++stage) {
-                s = wy_gemm(L, stage, stage >= 4 ? 0.5 * L.flops : 0.0);
+                s = wy_gemm(L, stage, stage >= WY_STAGES - 2 ? 0.5 * L.flops : 0.0);
                 if (s) return s;
             }
         }
