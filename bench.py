@@ -45,6 +45,7 @@ PEAK_HBM_GBS = 8000.0
 # bulge-chasing back-transformation (k_sbr_bt, 2 d^3) on the FP64 FMA units
 # (the same 78.6 TFLOP/s peak); the bulge chasing itself (k_sbr_chase) is
 # bound by the latency of its sequential tasks and is not priced here.
+FP64_VECTOR = {"k_sbr_bt"}  # compute-priced kernels that run on the FP64 FMA units, not MFMA
 KERNEL_BOUND = {"k_eig_tri_blk": "hbm", "k_eig_tri_coop": "hbm", "k_spec_bin": "hbm", "k_assemble": "hbm",
                 "k_gather_scatter": "hbm", "k_zero": "hbm", "k_eig_copy": "hbm", "k_gemm": "mfma",
                 "k_gemm_dc": "mfma", "k_gemm_bt": "mfma", "k_sbr_bt": "mfma"}
@@ -231,6 +232,9 @@ def main():
             ach = k["flops"] / (k["ms"] * 1e-3) / 1e12
             roof = {"kernel": name, "bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                     "frac": ach / PEAK_FP64_TFLOPS}
+            if name in FP64_VECTOR:
+                roof["compute"] = ("FP64 vector FMA, no MFMA instructions; the peak is MI355X's FP64 rate, "
+                                   "78.6 TFLOP/s for vector and matrix alike")
         else:
             ach = k["bytes"] / (k["ms"] * 1e-3) / 1e9
             roof = {"kernel": name, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
